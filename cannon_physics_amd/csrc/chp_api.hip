@@ -1,0 +1,785 @@
+// chp_api.hip -- kernels and the C-ABI of libcannon_hip.so (include/cannon_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cannon_hip.h"
+#include "chp_build.h"
+#include "chp_stages.h"
+
+using namespace chp;
+
+// ---------------------------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------------------------
+
+// batched path: one workgroup per listed world, nsteps whole steps per launch
+template <typename R>
+__global__ __launch_bounds__(CHP_BLOCK) void k_step_fused(const DevWorld<R>* __restrict__ worlds,
+                                                          const int* __restrict__ list, int nsteps, double dt) {
+  __shared__ int sh[CHP_BLOCK];
+  __shared__ R shr[CHP_BLOCK];
+  const DevWorld<R>& W = worlds[list[blockIdx.x]];
+  for (int s = 0; s < nsteps; s++)
+    if (!world_step<R>(W, (R)dt, sh, shr)) break;
+}
+
+// plain path: one stage of one world per launch
+enum { ST_PRE = 0, ST_BP, ST_FILTER, ST_NARROW, ST_ROWS, ST_LEVELS, ST_GS, ST_POST };
+template <typename R, int STAGE>
+__global__ __launch_bounds__(CHP_BLOCK) void k_stage(const DevWorld<R>* __restrict__ worlds, int wid, double dt) {
+  __shared__ int sh[CHP_BLOCK];
+  __shared__ R shr[CHP_BLOCK];
+  const DevWorld<R>& W = worlds[wid];
+  if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
+  R h = (R)dt;
+  switch (STAGE) {
+    case ST_PRE:
+      if (threadIdx.x == 0) W.counters[C_STAGING] = 0;
+      st_prestep(W);
+      break;
+    case ST_BP: W.broadphase == CHP_BROADPHASE_SAP ? st_bp_sap(W, sh) : st_bp_naive(W, sh); break;
+    case ST_FILTER: st_filter(W); break;
+    case ST_NARROW: st_narrow(W, h); break;
+    case ST_ROWS: st_rows(W, h, sh); break;
+    case ST_LEVELS: st_levels(W); break;
+    case ST_GS: st_gs(W, shr); break;
+    case ST_POST: st_post(W, h); break;
+  }
+}
+
+// broadphase only (Broadphase.collisionPairs on the current state)
+template <typename R>
+__global__ __launch_bounds__(CHP_BLOCK) void k_broadphase(const DevWorld<R>* __restrict__ worlds, int wid) {
+  __shared__ int sh[CHP_BLOCK];
+  const DevWorld<R>& W = worlds[wid];
+  if (W.broadphase == CHP_BROADPHASE_SAP) {
+    for (int i = threadIdx.x; i < W.nb; i += blockDim.x)
+      if (W.flags[i] & BF_AABB_DIRTY) { body_aabb(W, i); W.flags[i] &= ~BF_AABB_DIRTY; }
+    __syncthreads();
+    st_bp_sap(W, sh);
+  } else {
+    st_bp_naive(W, sh);
+  }
+}
+
+__global__ void k_gather_status(const int* const* counters, int n, int* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    for (int k = 0; k < C_COUNT; k++) out[i * C_COUNT + k] = counters[i][k];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------------
+
+namespace {
+
+struct Slab {  // one device allocation carved into 256-byte aligned arrays
+  size_t size = 0;
+  std::vector<std::pair<size_t, std::vector<char>>> init;  // offset, bytes to upload
+  template <typename T> size_t add(size_t n) {
+    size_t off = (size + 255) & ~(size_t)255;
+    size = off + std::max<size_t>(1, n) * sizeof(T);
+    return off;
+  }
+  template <typename T> size_t add(const std::vector<T>& v) {
+    size_t off = add<T>(v.size());
+    if (!v.empty()) {
+      std::vector<char> b(v.size() * sizeof(T));
+      memcpy(b.data(), v.data(), b.size());
+      init.emplace_back(off, std::move(b));
+    }
+    return off;
+  }
+};
+
+struct WorldSlot {
+  bool live = false;
+  int nb = 0;
+  void* mem = nullptr;
+  size_t bytes = 0;
+  double last_dt = -1;
+  std::vector<double> lin_damp, ang_damp;  // Body.linearDamping / angularDamping
+  std::vector<chp_body_info> info;
+  int pair_cap = 0, contact_cap = 0, row_cap = 0;
+  size_t off_ldamp = 0, off_adamp = 0, off_counters = 0;
+  std::vector<char> devworld;  // host image of DevWorld<R>
+  std::shared_ptr<void> host;  // HostWorld<R>: static tables, reused when the slabs are resized
+};
+
+}  // namespace
+
+struct chp_ctx {
+  int device = 0;
+  int precision = CHP_PRECISION_F32;
+  uint64_t budget = 0;
+  uint64_t in_use = 0;
+  hipStream_t stream = nullptr;
+  std::vector<WorldSlot> worlds;
+  void* d_worlds = nullptr;   // DevWorld<R>[capacity]
+  size_t d_worlds_cap = 0;
+  bool worlds_dirty = true;
+  int* d_list = nullptr;
+  size_t d_list_cap = 0;
+  int* d_status = nullptr;
+  const int** d_cptrs = nullptr;
+  size_t d_status_cap = 0;
+  std::string err;
+  std::mutex mu;
+};
+
+static int fail(chp_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail(ctx, CHP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename R> static size_t devworld_bytes() { return sizeof(DevWorld<R>); }
+
+// sizes of the per-world scratch slabs
+static void default_caps(int nb, int nhf, int& pairs, int& contacts, int& rows) {
+  long long p = 48LL * nb + 256;
+  long long c = 12LL * nb + 256 + (nhf ? 64LL * nb : 0);
+  pairs = (int)std::min<long long>(p, 1LL << 30);
+  contacts = (int)std::min<long long>(c, 1LL << 30);
+  rows = (int)std::min<long long>(3 * c + 64, 1LL << 30);
+}
+
+template <typename R>
+static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pair_cap, int contact_cap, int row_cap) {
+  Slab L;
+  DevWorld<R> D;
+  memset(&D, 0, sizeof(D));
+  int nb = H.nb;
+  D.nb = nb;
+  D.nsi = (int)H.si_type.size();
+  D.ncons = (int)H.c_type.size();
+  D.ncrows = H.ncrows;
+  D.pair_cap = pair_cap;
+  D.contact_cap = contact_cap;
+  D.row_cap = row_cap;
+  D.nmat = H.nmat;
+  D.broadphase = H.wd.broadphase;
+  D.sap_axis = H.sap_axis;
+  D.iterations = H.wd.iterations;
+  D.quat_skip = H.wd.quat_normalize_skip;
+  D.quat_fast = H.wd.quat_normalize_fast;
+  D.has_filtered_constraints = H.has_filtered ? 1 : 0;
+  for (int k = 0; k < 3; k++) D.gravity[k] = H.wd.gravity[k];
+  {
+    f3 g = H.wd.has_friction_gravity ? mk3(H.wd.friction_gravity[0], H.wd.friction_gravity[1], H.wd.friction_gravity[2])
+                                     : mk3(H.wd.gravity[0], H.wd.gravity[1], H.wd.gravity[2]);
+    D.fric_grav_len = len3<R>(g);
+  }
+  D.tol2 = (R)H.wd.tolerance * (R)H.wd.tolerance;
+  {  // Equation constructor: setSpookParams(1e7, 4, 1/60)
+    R d = 4, k = (R)1e7, h = (R)1 / (R)60;
+    D.c_spook[0] = (R)4 / (h * ((R)1 + (R)4 * d));
+    D.c_spook[1] = ((R)4 * d) / ((R)1 + (R)4 * d);
+    D.c_spook[2] = (R)4 / (h * h * k * ((R)1 + (R)4 * d));
+  }
+  D.nhf = (int)H.hf_nx.size();
+  // record offsets; pointers are patched once the slab exists
+  std::vector<std::pair<void**, size_t>> fix;
+#define ARR(field, vec) fix.emplace_back((void**)&D.field, L.add(vec))
+#define SCR(field, T, n) fix.emplace_back((void**)&D.field, L.add<T>(n))
+  ARR(pos, H.pos); ARR(quat, H.quat); ARR(vel, H.vel); ARR(avel, H.avel); ARR(force, H.force); ARR(torque, H.torque);
+  ARR(iiw, H.iiw); ARR(inv_inertia, H.inv_inertia); ARR(lin_factor, H.lin_factor); ARR(ang_factor, H.ang_factor);
+  ARR(aabb_lo, H.aabb_lo); ARR(aabb_hi, H.aabb_hi);
+  ARR(mass, H.mass); ARR(inv_mass, H.inv_mass); ARR(brad, H.brad);
+  size_t o_ld = L.add<R>(nb), o_ad = L.add<R>(nb);
+  fix.emplace_back((void**)&D.ldamp, o_ld);
+  fix.emplace_back((void**)&D.adamp, o_ad);
+  ARR(type, H.type); ARR(flags, H.flags); ARR(group, H.group); ARR(mask, H.mask); ARR(material, H.material);
+  ARR(shape_first, H.shape_first); ARR(shape_count, H.shape_count); ARR(order, H.order);
+  ARR(si_type, H.si_type); ARR(si_hull, H.si_hull); ARR(si_hf, H.si_hf); ARR(si_response, H.si_response);
+  ARR(si_group, H.si_group); ARR(si_mask, H.si_mask); ARR(si_material, H.si_material);
+  ARR(si_off, H.si_off); ARR(si_ori, H.si_ori); ARR(si_half, H.si_half); ARR(si_radius, H.si_radius); ARR(si_bsr, H.si_bsr);
+  ARR(h_vstart, H.H.vstart); ARR(h_vcount, H.H.vcount); ARR(h_fstart, H.H.fstart); ARR(h_fcount, H.H.fcount);
+  ARR(h_estart, H.H.estart); ARR(h_ecount, H.H.ecount); ARR(h_axes, H.H.axes);
+  std::vector<R> hbsr(H.H.bsr.begin(), H.H.bsr.end());
+  ARR(h_bsr, hbsr);
+  ARR(h_verts, H.H.verts); ARR(h_fnormal, H.H.fnormal); ARR(h_edges, H.H.edges);
+  ARR(f_istart, H.H.f_istart); ARR(f_idx, H.H.f_idx);
+  ARR(hf_nx, H.hf_nx); ARR(hf_ny, H.hf_ny); ARR(hf_dstart, H.hf_dstart); ARR(hf_pillar_hull, H.hf_pillar_hull);
+  ARR(hf_min, H.hf_min); ARR(hf_max, H.hf_max); ARR(hf_es, H.hf_es); ARR(hf_data, H.hf_data);
+  ARR(hf_pillar_off, H.hf_pillar_off);
+  ARR(mat_friction, H.mat_friction); ARR(mat_restitution, H.mat_restitution); ARR(cm_table, H.cm_table);
+  ARR(cm_params, H.cm_params);
+  ARR(c_type, H.c_type); ARR(c_a, H.c_a); ARR(c_b, H.c_b); ARR(c_collide, H.c_collide); ARR(c_row_off, H.c_row_off);
+  ARR(c_eq_mask, H.c_eq_mask); ARR(c_vec, H.c_vec); ARR(c_eq, H.c_eq);
+  SCR(feff, float, 3 * nb); SCR(vl, float, 3 * nb); SCR(wl, float, 3 * nb); SCR(iiws, float, 9 * nb); SCR(ims, R, nb);
+  int np2 = 1;
+  while (np2 < nb) np2 <<= 1;
+  SCR(pair_i, int, pair_cap); SCR(pair_j, int, pair_cap); SCR(cnt, int, nb); SCR(tmp, int, nb);
+  SCR(pc_cnt, int, pair_cap); SCR(pc_base, int, pair_cap); SCR(pf_cnt, int, pair_cap); SCR(pc_off, int, pair_cap);
+  SCR(pf_off, int, 1);
+  SCR(contacts, ContactRec<R>, contact_cap);
+  SCR(rows, RowRec<R>, row_cap);
+  SCR(level, int, row_cap); SCR(sched, int, row_cap); SCR(lvl_start, int, row_cap + 2);
+  SCR(last_level, int, std::max(nb, row_cap + 1));
+  SCR(sort_keys, unsigned long long, np2);
+  size_t o_cnt = L.add<int>(C_COUNT);
+  fix.emplace_back((void**)&D.counters, o_cnt);
+#undef ARR
+#undef SCR
+  if (ctx->in_use + L.size > ctx->budget)
+    return fail(ctx, CHP_E_OOM, "world needs " + std::to_string(L.size) + " bytes; context budget " +
+                                    std::to_string(ctx->budget) + ", in use " + std::to_string(ctx->in_use));
+  void* mem = nullptr;
+  if (hipMalloc(&mem, L.size) != hipSuccess) return fail(ctx, CHP_E_OOM, "hipMalloc of a world slab failed");
+  std::vector<char> host(L.size, 0);
+  for (auto& it : L.init) memcpy(host.data() + it.first, it.second.data(), it.second.size());
+  hipError_t e = hipMemcpy(mem, host.data(), L.size, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { hipFree(mem); return fail(ctx, CHP_E_HIP, std::string("upload: ") + hipGetErrorString(e)); }
+  for (auto& f : fix) *f.first = (char*)mem + f.second;
+  if (S.mem) { hipFree(S.mem); ctx->in_use -= S.bytes; }
+  S.mem = mem;
+  S.bytes = L.size;
+  ctx->in_use += L.size;
+  S.nb = nb;
+  S.pair_cap = pair_cap; S.contact_cap = contact_cap; S.row_cap = row_cap;
+  S.off_ldamp = o_ld; S.off_adamp = o_ad; S.off_counters = o_cnt;
+  S.last_dt = -1;
+  S.devworld.assign((char*)&D, (char*)&D + sizeof(D));
+  ctx->worlds_dirty = true;
+  return CHP_OK;
+}
+
+template <typename R> static int world_create_t(chp_ctx* ctx, const chp_scene* scene, uint32_t* id) {
+  HostWorld<R> H;
+  try {
+    Builder<R> B(H, *scene);
+    B.run();
+  } catch (const std::exception& ex) {
+    return fail(ctx, CHP_E_INVALID, ex.what());
+  }
+  WorldSlot S;
+  S.live = true;
+  for (uint32_t i = 0; i < scene->num_bodies; i++) {
+    S.lin_damp.push_back(scene->bodies[i].linear_damping);
+    S.ang_damp.push_back(scene->bodies[i].angular_damping);
+    chp_body_info bi;
+    memset(&bi, 0, sizeof(bi));
+    bi.mass = (double)H.mass[i]; bi.inv_mass = (double)H.inv_mass[i]; bi.bounding_radius = (double)H.brad[i];
+    for (int k = 0; k < 3; k++) { bi.inertia[k] = H.inertia[3 * i + k]; bi.inv_inertia[k] = H.inv_inertia[3 * i + k]; }
+    bi.type = H.type[i];
+    S.info.push_back(bi);
+  }
+  int pc, cc, rc;
+  default_caps(H.nb, (int)H.hf_nx.size(), pc, cc, rc);
+  rc = std::max(rc, H.ncrows + 3 * cc);
+  int r = build_slot<R>(ctx, S, H, pc, cc, rc);
+  if (r != CHP_OK) return r;
+  S.host = std::make_shared<HostWorld<R>>(std::move(H));
+  size_t slot = ctx->worlds.size();
+  for (size_t k = 0; k < ctx->worlds.size(); k++)
+    if (!ctx->worlds[k].live) { slot = k; break; }
+  if (slot == ctx->worlds.size()) ctx->worlds.emplace_back();
+  ctx->worlds[slot] = std::move(S);
+  *id = (uint32_t)slot;
+  return CHP_OK;
+}
+
+static bool valid_world(chp_ctx* ctx, uint32_t id) { return ctx && id < ctx->worlds.size() && ctx->worlds[id].live; }
+
+template <typename R> static DevWorld<R>& dev_of(WorldSlot& S) { return *(DevWorld<R>*)S.devworld.data(); }
+
+// uploads the DevWorld table and the damping factors for dt
+template <typename R> static int prepare(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt) {
+  if (ctx->worlds_dirty || ctx->d_worlds_cap < ctx->worlds.size()) {
+    size_t cap = std::max<size_t>(ctx->worlds.size(), 1);
+    if (ctx->d_worlds_cap < cap) {
+      if (ctx->d_worlds) hipFree(ctx->d_worlds);
+      HIPCHK(ctx, hipMalloc(&ctx->d_worlds, cap * sizeof(DevWorld<R>)));
+      ctx->d_worlds_cap = cap;
+    }
+    std::vector<DevWorld<R>> tab(ctx->worlds.size());
+    for (size_t k = 0; k < ctx->worlds.size(); k++)
+      if (ctx->worlds[k].live) tab[k] = dev_of<R>(ctx->worlds[k]);
+      else memset(&tab[k], 0, sizeof(DevWorld<R>));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_worlds, tab.data(), tab.size() * sizeof(DevWorld<R>), hipMemcpyHostToDevice, ctx->stream));
+    ctx->worlds_dirty = false;
+  }
+  for (uint32_t k = 0; k < n; k++) {
+    WorldSlot& S = ctx->worlds[ids[k]];
+    if (S.last_dt == dt) continue;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    // world_class.dart:652-656: pow(1 - damping, dt), evaluated once per (body, dt)
+    std::vector<R> ldv(S.nb), adv(S.nb);
+    for (int i = 0; i < S.nb; i++) {
+      ldv[i] = (R)std::pow((double)((R)1 - (R)S.lin_damp[i]), (double)(R)dt);
+      adv[i] = (R)std::pow((double)((R)1 - (R)S.ang_damp[i]), (double)(R)dt);
+    }
+    HIPCHK(ctx, hipMemcpy((char*)S.mem + S.off_ldamp, ldv.data(), S.nb * sizeof(R), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy((char*)S.mem + S.off_adamp, adv.data(), S.nb * sizeof(R), hipMemcpyHostToDevice));
+    S.last_dt = dt;
+  }
+  return CHP_OK;
+}
+
+static int read_status(chp_ctx* ctx, const uint32_t* ids, uint32_t n, std::vector<int>& out) {
+  if (ctx->d_status_cap < n) {
+    if (ctx->d_status) hipFree(ctx->d_status);
+    if (ctx->d_cptrs) hipFree(ctx->d_cptrs);
+    HIPCHK(ctx, hipMalloc(&ctx->d_status, (size_t)n * C_COUNT * sizeof(int)));
+    HIPCHK(ctx, hipMalloc(&ctx->d_cptrs, (size_t)n * sizeof(int*)));
+    ctx->d_status_cap = n;
+  }
+  std::vector<const int*> ptrs(n);
+  for (uint32_t k = 0; k < n; k++) ptrs[k] = (const int*)((char*)ctx->worlds[ids[k]].mem + ctx->worlds[ids[k]].off_counters);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_cptrs, ptrs.data(), n * sizeof(int*), hipMemcpyHostToDevice, ctx->stream));
+  k_gather_status<<<(n + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_cptrs, (int)n, ctx->d_status);
+  HIPCHK(ctx, hipGetLastError());
+  out.resize((size_t)n * C_COUNT);
+  HIPCHK(ctx, hipMemcpyAsync(out.data(), ctx->d_status, out.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CHP_OK;
+}
+
+static int check_status(chp_ctx* ctx, const uint32_t* ids, uint32_t n) {
+  std::vector<int> st;
+  int r = read_status(ctx, ids, n, st);
+  if (r != CHP_OK) return r;
+  for (uint32_t k = 0; k < n; k++) {
+    int s = st[k * C_COUNT + C_STATUS];
+    if (!s) continue;
+    WorldSlot& S = ctx->worlds[ids[k]];
+    char buf[256];
+    if (s & WS_NONFINITE) {
+      snprintf(buf, sizeof buf, "world %u produced non-finite state and is quarantined", ids[k]);
+      return fail(ctx, CHP_E_NONFINITE, buf);
+    }
+    snprintf(buf, sizeof buf,
+             "world %u exceeded a slab (status %d): pairs %d/%d, contacts %d/%d, rows %d/%d; raise it with "
+             "chp_world_set_capacity",
+             ids[k], s, st[k * C_COUNT + C_PAIRS], S.pair_cap, st[k * C_COUNT + C_STAGING], S.contact_cap,
+             st[k * C_COUNT + C_ROWS], S.row_cap);
+    return fail(ctx, CHP_E_CAPACITY, buf);
+  }
+  return CHP_OK;
+}
+
+template <typename R> static int step_t(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt, int nsteps, bool wait) {
+  int r = prepare<R>(ctx, ids, n, dt);
+  if (r != CHP_OK) return r;
+  if (ctx->d_list_cap < n) {
+    if (ctx->d_list) hipFree(ctx->d_list);
+    HIPCHK(ctx, hipMalloc(&ctx->d_list, (size_t)n * sizeof(int)));
+    ctx->d_list_cap = n;
+  }
+  // largest worlds first so that the long workgroups start early
+  std::vector<int> list(ids, ids + n);
+  std::stable_sort(list.begin(), list.end(), [&](int a, int b) { return ctx->worlds[a].nb > ctx->worlds[b].nb; });
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, list.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  k_step_fused<R><<<n, CHP_BLOCK, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, ctx->d_list, nsteps, dt);
+  HIPCHK(ctx, hipGetLastError());
+  if (!wait) return CHP_OK;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return check_status(ctx, ids, n);
+}
+
+template <typename R> static int step_plain_t(chp_ctx* ctx, uint32_t id, double dt, int nsteps) {
+  int r = prepare<R>(ctx, &id, 1, dt);
+  if (r != CHP_OK) return r;
+  const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
+  for (int s = 0; s < nsteps; s++) {
+    k_stage<R, ST_PRE><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
+    k_stage<R, ST_BP><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
+    k_stage<R, ST_FILTER><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
+    k_stage<R, ST_NARROW><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
+    k_stage<R, ST_ROWS><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
+    k_stage<R, ST_LEVELS><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
+    k_stage<R, ST_GS><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
+    k_stage<R, ST_POST><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return check_status(ctx, &id, 1);
+}
+
+template <typename R> static int set_capacity_t(chp_ctx* ctx, uint32_t id, uint32_t mp, uint32_t mc, uint32_t mr) {
+  WorldSlot& S = ctx->worlds[id];
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HostWorld<R>& H = *(HostWorld<R>*)S.host.get();
+  DevWorld<R>& D = dev_of<R>(S);
+  int nb = S.nb;
+  // the mutable per-body arrays come back from the device; everything else is static
+  auto get = [&](auto& vec, const void* src) {
+    return hipMemcpy(vec.data(), src, vec.size() * sizeof(vec[0]), hipMemcpyDeviceToHost);
+  };
+  hipError_t e = hipSuccess;
+#define G(field) if (e == hipSuccess) e = get(H.field, D.field)
+  G(pos); G(quat); G(vel); G(avel); G(force); G(torque); G(iiw); G(aabb_lo); G(aabb_hi); G(flags); G(order);
+#undef G
+  int counters[C_COUNT];
+  if (e == hipSuccess) e = hipMemcpy(counters, D.counters, sizeof counters, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(ctx, CHP_E_HIP, std::string("capacity change: ") + hipGetErrorString(e));
+  (void)nb;
+  int pc = mp ? (int)mp : S.pair_cap, cc = mc ? (int)mc : S.contact_cap, rc = mr ? (int)mr : S.row_cap;
+  int r = build_slot<R>(ctx, S, H, pc, cc, rc);
+  if (r != CHP_OK) return r;
+  DevWorld<R>& D2 = dev_of<R>(S);
+  int keep[C_COUNT] = {0};
+  keep[C_STEP] = counters[C_STEP];
+  HIPCHK(ctx, hipMemcpy(D2.counters, keep, sizeof keep, hipMemcpyHostToDevice));
+  return CHP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------------------------
+
+template <typename R> static int state_io(chp_ctx* ctx, uint32_t id, chp_body_state* buf, uint32_t n, bool download) {
+  WorldSlot& S = ctx->worlds[id];
+  DevWorld<R>& D = dev_of<R>(S);
+  if (n < (uint32_t)S.nb) return fail(ctx, CHP_E_CAPACITY, "state buffer smaller than the body count");
+  int nb = S.nb;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<float> pos(3 * nb), quat(4 * nb), vel(3 * nb), avel(3 * nb), force(3 * nb), torque(3 * nb), iiw(9 * nb),
+      lo(3 * nb), hi(3 * nb);
+  std::vector<int> flags(nb);
+  struct F { std::vector<float>* v; float* d; } fs[] = {{&pos, D.pos}, {&quat, D.quat}, {&vel, D.vel}, {&avel, D.avel},
+                                                       {&force, D.force}, {&torque, D.torque}, {&iiw, D.iiw},
+                                                       {&lo, D.aabb_lo}, {&hi, D.aabb_hi}};
+  if (download) {
+    for (auto& f : fs) HIPCHK(ctx, hipMemcpy(f.v->data(), f.d, f.v->size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(flags.data(), D.flags, nb * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; i++) {
+      chp_body_state& s = buf[i];
+      memset(&s, 0, sizeof(s));
+      memcpy(s.position, &pos[3 * i], 12); memcpy(s.quaternion, &quat[4 * i], 16);
+      memcpy(s.velocity, &vel[3 * i], 12); memcpy(s.angular_velocity, &avel[3 * i], 12);
+      memcpy(s.force, &force[3 * i], 12); memcpy(s.torque, &torque[3 * i], 12);
+      memcpy(s.inv_inertia_world, &iiw[9 * i], 36);
+      memcpy(s.aabb_lower, &lo[3 * i], 12); memcpy(s.aabb_upper, &hi[3 * i], 12);
+      s.aabb_needs_update = (flags[i] & BF_AABB_DIRTY) ? 1 : 0;
+    }
+  } else {
+    HIPCHK(ctx, hipMemcpy(flags.data(), D.flags, nb * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; i++) {
+      const chp_body_state& s = buf[i];
+      memcpy(&pos[3 * i], s.position, 12); memcpy(&quat[4 * i], s.quaternion, 16);
+      memcpy(&vel[3 * i], s.velocity, 12); memcpy(&avel[3 * i], s.angular_velocity, 12);
+      memcpy(&force[3 * i], s.force, 12); memcpy(&torque[3 * i], s.torque, 12);
+      memcpy(&iiw[9 * i], s.inv_inertia_world, 36);
+      memcpy(&lo[3 * i], s.aabb_lower, 12); memcpy(&hi[3 * i], s.aabb_upper, 12);
+      flags[i] = (flags[i] & ~BF_AABB_DIRTY) | (s.aabb_needs_update ? BF_AABB_DIRTY : 0);
+    }
+    for (auto& f : fs) HIPCHK(ctx, hipMemcpy(f.d, f.v->data(), f.v->size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(D.flags, flags.data(), nb * 4, hipMemcpyHostToDevice));
+    int zero = 0;  // a fresh state clears a quarantine
+    HIPCHK(ctx, hipMemcpy(D.counters + C_STATUS, &zero, 4, hipMemcpyHostToDevice));
+  }
+  return CHP_OK;
+}
+
+template <typename R> static int broadphase_t(chp_ctx* ctx, uint32_t id, int32_t* pi, int32_t* pj, uint32_t cap, uint32_t* np) {
+  int r = prepare<R>(ctx, &id, 1, ctx->worlds[id].last_dt > 0 ? ctx->worlds[id].last_dt : 1.0 / 60);
+  if (r != CHP_OK) return r;
+  k_broadphase<R><<<1, CHP_BLOCK, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, (int)id);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  DevWorld<R>& D = dev_of<R>(ctx->worlds[id]);
+  int c[C_COUNT];
+  HIPCHK(ctx, hipMemcpy(c, D.counters, sizeof c, hipMemcpyDeviceToHost));
+  if (c[C_STATUS] & WS_PAIRS) {
+    int zero = 0;
+    HIPCHK(ctx, hipMemcpy(D.counters + C_STATUS, &zero, 4, hipMemcpyHostToDevice));
+    return fail(ctx, CHP_E_CAPACITY, "pair slab too small: " + std::to_string(c[C_PAIRS]) + " pairs");
+  }
+  *np = (uint32_t)c[C_PAIRS];
+  if (*np > cap) return fail(ctx, CHP_E_CAPACITY, "pair buffers too small: need " + std::to_string(*np));
+  HIPCHK(ctx, hipMemcpy(pi, D.pair_i, *np * 4, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(pj, D.pair_j, *np * 4, hipMemcpyDeviceToHost));
+  return CHP_OK;
+}
+
+template <typename R>
+static int capture_t(chp_ctx* ctx, uint32_t id, double dt, int32_t* pi, int32_t* pj, uint32_t pcap, uint32_t* np,
+                     chp_row* rows, uint32_t rcap, uint32_t* nr, int32_t* iters) {
+  int r = step_plain_t<R>(ctx, id, dt, 1);
+  if (r != CHP_OK) return r;
+  DevWorld<R>& D = dev_of<R>(ctx->worlds[id]);
+  int c[C_COUNT];
+  HIPCHK(ctx, hipMemcpy(c, D.counters, sizeof c, hipMemcpyDeviceToHost));
+  int P = c[C_PAIRS], E = c[C_ROWS];
+  std::vector<int> a(P), b(P);
+  if (P) {
+    HIPCHK(ctx, hipMemcpy(a.data(), D.pair_i, P * 4, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(b.data(), D.pair_j, P * 4, hipMemcpyDeviceToHost));
+  }
+  uint32_t k = 0;
+  for (int p = 0; p < P; p++) {
+    if (a[p] < 0) continue;  // dropped by collideConnected
+    if (k < pcap) { pi[k] = a[p]; pj[k] = b[p]; }
+    k++;
+  }
+  *np = k;
+  std::vector<RowRec<R>> rr(E);
+  if (E) HIPCHK(ctx, hipMemcpy(rr.data(), D.rows, E * sizeof(RowRec<R>), hipMemcpyDeviceToHost));
+  *nr = (uint32_t)E;
+  for (int e = 0; e < E && (uint32_t)e < rcap; e++) {
+    chp_row& o = rows[e];
+    memset(&o, 0, sizeof(o));
+    o.kind = rr[e].kind; o.body_i = rr[e].a; o.body_j = rr[e].b;
+    memcpy(o.ga_spatial, &rr[e].sa, 12); memcpy(o.ga_rotational, &rr[e].ra, 12);
+    memcpy(o.gb_spatial, &rr[e].sb, 12); memcpy(o.gb_rotational, &rr[e].rb, 12);
+    o.b = (double)rr[e].B; o.inv_c = (double)rr[e].invC; o.eps = (double)rr[e].eps;
+    o.min_force = (double)rr[e].minF; o.max_force = (double)rr[e].maxF; o.lambda = (double)rr[e].lambda;
+  }
+  *iters = c[C_ITERS];
+  if (k > pcap || (uint32_t)E > rcap) return fail(ctx, CHP_E_CAPACITY, "capture buffers too small");
+  return CHP_OK;
+}
+
+
+template <typename R> static int inspect_t(const chp_scene* scene, chp_body_info* info, chp_body_state* state,
+                                           uint32_t cap, char* err, uint32_t errcap) {
+  HostWorld<R> H;
+  try {
+    Builder<R> B(H, *scene);
+    B.run();
+  } catch (const std::exception& ex) {
+    if (err && errcap) { strncpy(err, ex.what(), errcap - 1); err[errcap - 1] = 0; }
+    return CHP_E_INVALID;
+  }
+  if (cap < (uint32_t)H.nb) return CHP_E_CAPACITY;
+  for (int i = 0; i < H.nb; i++) {
+    if (info) {
+      chp_body_info& b = info[i];
+      memset(&b, 0, sizeof(b));
+      b.mass = (double)H.mass[i]; b.inv_mass = (double)H.inv_mass[i]; b.bounding_radius = (double)H.brad[i];
+      for (int k = 0; k < 3; k++) { b.inertia[k] = H.inertia[3 * i + k]; b.inv_inertia[k] = H.inv_inertia[3 * i + k]; }
+      b.type = H.type[i];
+    }
+    if (state) {
+      chp_body_state& s = state[i];
+      memset(&s, 0, sizeof(s));
+      memcpy(s.position, &H.pos[3 * i], 12); memcpy(s.quaternion, &H.quat[4 * i], 16);
+      memcpy(s.velocity, &H.vel[3 * i], 12); memcpy(s.angular_velocity, &H.avel[3 * i], 12);
+      memcpy(s.force, &H.force[3 * i], 12); memcpy(s.torque, &H.torque[3 * i], 12);
+      memcpy(s.inv_inertia_world, &H.iiw[9 * i], 36);
+      memcpy(s.aabb_lower, &H.aabb_lo[3 * i], 12); memcpy(s.aabb_upper, &H.aabb_hi[3 * i], 12);
+      s.aabb_needs_update = (H.flags[i] & BF_AABB_DIRTY) ? 1 : 0;
+    }
+  }
+  return CHP_OK;
+}
+
+extern "C" {
+
+int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out) {
+  if (!out) return CHP_E_INVALID;
+  *out = nullptr;
+  if (precision != CHP_PRECISION_F32 && precision != CHP_PRECISION_F64) return CHP_E_INVALID;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return CHP_E_NODEVICE;
+  if (hipSetDevice(device) != hipSuccess) return CHP_E_NODEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return CHP_E_NODEVICE;
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) return CHP_E_NODEVICE;
+  chp_ctx* c = new chp_ctx();
+  c->device = device;
+  c->precision = precision;
+  size_t fr = 0, tot = 0;
+  hipMemGetInfo(&fr, &tot);
+  c->budget = hbm_budget ? hbm_budget : (uint64_t)(0.9 * (double)fr);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CHP_E_HIP; }
+  *out = c;
+  return CHP_OK;
+}
+
+int chp_ctx_destroy(chp_ctx* ctx) {
+  if (!ctx) return CHP_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& w : ctx->worlds) if (w.mem) hipFree(w.mem);
+  if (ctx->d_worlds) hipFree(ctx->d_worlds);
+  if (ctx->d_list) hipFree(ctx->d_list);
+  if (ctx->d_status) hipFree(ctx->d_status);
+  if (ctx->d_cptrs) hipFree(ctx->d_cptrs);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CHP_OK;
+}
+
+const char* chp_last_error(const chp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+uint64_t chp_ctx_bytes_in_use(const chp_ctx* ctx) { return ctx ? ctx->in_use : 0; }
+
+int chp_world_create(chp_ctx* ctx, const chp_scene* scene, uint32_t* world_id) {
+  if (!ctx || !scene || !world_id) return fail(ctx, CHP_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  if (scene->num_bodies == 0) return fail(ctx, CHP_E_INVALID, "a world needs at least one body");
+  return ctx->precision == CHP_PRECISION_F64 ? world_create_t<double>(ctx, scene, world_id)
+                                             : world_create_t<float>(ctx, scene, world_id);
+}
+
+int chp_world_destroy(chp_ctx* ctx, uint32_t id) {
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  WorldSlot& S = ctx->worlds[id];
+  hipFree(S.mem);
+  ctx->in_use -= S.bytes;
+  S = WorldSlot();
+  ctx->worlds_dirty = true;
+  return CHP_OK;
+}
+
+int chp_world_num_bodies(chp_ctx* ctx, uint32_t id, uint32_t* n) {
+  if (!valid_world(ctx, id) || !n) return fail(ctx, CHP_E_INVALID, "unknown world");
+  *n = (uint32_t)ctx->worlds[id].nb;
+  return CHP_OK;
+}
+
+int chp_world_body_info(chp_ctx* ctx, uint32_t id, chp_body_info* out, uint32_t cap) {
+  if (!valid_world(ctx, id) || !out) return fail(ctx, CHP_E_INVALID, "unknown world");
+  WorldSlot& S = ctx->worlds[id];
+  if (cap < (uint32_t)S.nb) return fail(ctx, CHP_E_CAPACITY, "body_info buffer too small");
+  memcpy(out, S.info.data(), S.nb * sizeof(chp_body_info));
+  return CHP_OK;
+}
+
+int chp_world_set_capacity(chp_ctx* ctx, uint32_t id, uint32_t mp, uint32_t mc, uint32_t mr) {
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? set_capacity_t<double>(ctx, id, mp, mc, mr)
+                                             : set_capacity_t<float>(ctx, id, mp, mc, mr);
+}
+
+
+int chp_state_download(chp_ctx* ctx, uint32_t id, chp_body_state* out, uint32_t cap) {
+  if (!valid_world(ctx, id) || !out) return fail(ctx, CHP_E_INVALID, "unknown world");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? state_io<double>(ctx, id, out, cap, true)
+                                             : state_io<float>(ctx, id, out, cap, true);
+}
+
+int chp_state_upload(chp_ctx* ctx, uint32_t id, const chp_body_state* in, uint32_t n) {
+  if (!valid_world(ctx, id) || !in) return fail(ctx, CHP_E_INVALID, "unknown world");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? state_io<double>(ctx, id, (chp_body_state*)in, n, false)
+                                             : state_io<float>(ctx, id, (chp_body_state*)in, n, false);
+}
+
+static int step_common(chp_ctx* ctx, const uint32_t* worlds, uint32_t n, double dt, int32_t nsteps, bool wait) {
+  if (!ctx || (!worlds && n)) return fail(ctx, CHP_E_INVALID, "null argument");
+  if (!(dt > 0) || nsteps < 0) return fail(ctx, CHP_E_INVALID, "dt must be > 0 and nsteps >= 0");
+  for (uint32_t k = 0; k < n; k++)
+    if (!valid_world(ctx, worlds[k])) return fail(ctx, CHP_E_INVALID, "unknown world id " + std::to_string(worlds[k]));
+  if (n == 0 || nsteps == 0) return CHP_OK;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? step_t<double>(ctx, worlds, n, dt, nsteps, wait)
+                                             : step_t<float>(ctx, worlds, n, dt, nsteps, wait);
+}
+
+int chp_step(chp_ctx* ctx, const uint32_t* worlds, uint32_t n, double dt, int32_t nsteps) {
+  return step_common(ctx, worlds, n, dt, nsteps, true);
+}
+int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t n, double dt, int32_t nsteps) {
+  return step_common(ctx, worlds, n, dt, nsteps, false);
+}
+int chp_sync(chp_ctx* ctx) {
+  if (!ctx) return CHP_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CHP_OK;
+}
+
+int chp_step_plain(chp_ctx* ctx, uint32_t id, double dt, int32_t nsteps) {
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!(dt > 0) || nsteps < 0) return fail(ctx, CHP_E_INVALID, "dt must be > 0 and nsteps >= 0");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? step_plain_t<double>(ctx, id, dt, nsteps)
+                                             : step_plain_t<float>(ctx, id, dt, nsteps);
+}
+
+int chp_world_stats(chp_ctx* ctx, uint32_t id, chp_step_stats* out) {
+  if (!valid_world(ctx, id) || !out) return fail(ctx, CHP_E_INVALID, "unknown world");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  std::vector<int> st;
+  int r = read_status(ctx, &id, 1, st);
+  if (r != CHP_OK) return r;
+  out->pairs = st[C_PAIRS];
+  out->contacts = st[C_CONTACTS];
+  out->rows = st[C_ROWS];
+  out->iterations = st[C_ITERS];
+  out->levels = st[C_LEVELS];
+  out->status = st[C_STATUS];
+  out->steps = st[C_STEP];
+  return CHP_OK;
+}
+
+
+int chp_broadphase(chp_ctx* ctx, uint32_t id, int32_t* pi, int32_t* pj, uint32_t cap, uint32_t* np) {
+  if (!valid_world(ctx, id) || !pi || !pj || !np) return fail(ctx, CHP_E_INVALID, "bad argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? broadphase_t<double>(ctx, id, pi, pj, cap, np)
+                                             : broadphase_t<float>(ctx, id, pi, pj, cap, np);
+}
+
+
+int chp_step_capture(chp_ctx* ctx, uint32_t id, double dt, int32_t* pi, int32_t* pj, uint32_t pair_cap, uint32_t* npairs,
+                     chp_row* rows, uint32_t row_cap, uint32_t* nrows, int32_t* iterations) {
+  if (!valid_world(ctx, id) || !pi || !pj || !npairs || !rows || !nrows || !iterations)
+    return fail(ctx, CHP_E_INVALID, "bad argument");
+  if (!(dt > 0)) return fail(ctx, CHP_E_INVALID, "dt must be > 0");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64
+             ? capture_t<double>(ctx, id, dt, pi, pj, pair_cap, npairs, rows, row_cap, nrows, iterations)
+             : capture_t<float>(ctx, id, dt, pi, pj, pair_cap, npairs, rows, row_cap, nrows, iterations);
+}
+
+void chp_quat_from_euler(double x, double y, double z, float out[4]) {
+  // Quat.setFromEuler, order xyz (quaternion.dart:191-232)
+  double c1 = std::cos(x / 2), c2 = std::cos(y / 2), c3 = std::cos(z / 2);
+  double s1 = std::sin(x / 2), s2 = std::sin(y / 2), s3 = std::sin(z / 2);
+  out[0] = (float)(s1 * c2 * c3 + c1 * s2 * s3);
+  out[1] = (float)(c1 * s2 * c3 - s1 * c2 * s3);
+  out[2] = (float)(c1 * c2 * s3 + s1 * s2 * c3);
+  out[3] = (float)(c1 * c2 * c3 - s1 * s2 * s3);
+}
+
+int chp_scene_inspect(const chp_scene* scene, int precision, chp_body_info* info, chp_body_state* state, uint32_t cap,
+                      char* err, uint32_t errcap) {
+  if (!scene) return CHP_E_INVALID;
+  if (precision == CHP_PRECISION_F64) return inspect_t<double>(scene, info, state, cap, err, errcap);
+  if (precision == CHP_PRECISION_F32) return inspect_t<float>(scene, info, state, cap, err, errcap);
+  return CHP_E_INVALID;
+}
+
+uint32_t chp_abi_struct_size(const char* name) {
+  std::string n = name ? name : "";
+#define SZ(t) if (n == #t) return (uint32_t)sizeof(t)
+  SZ(chp_world_desc); SZ(chp_material_desc); SZ(chp_contact_material_desc); SZ(chp_shape_desc); SZ(chp_hull_desc);
+  SZ(chp_heightfield_desc); SZ(chp_body_shape); SZ(chp_body_desc); SZ(chp_constraint_desc); SZ(chp_scene);
+  SZ(chp_body_state); SZ(chp_body_info); SZ(chp_row); SZ(chp_step_stats);
+#undef SZ
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+"""Seeded synthetic scenes for the benchmark workloads, built through the mirrored API.
+
+  c1_spheres        216 unit-mass spheres (r=0.5) on a 6x6x6 grid over a plane, Naive
+  c2_large_world    32,768 spheres/boxes in a 64x32x64 m volume over a plane, SAP (auto axis)
+  c3_world          one world of the 4096-world batch: a box stack or a sphere/box pile
+  c4_chain_world    one world of the 1024-world batch: a jointed chain of box links
+  c5_terrain_world  one serving request: spheres/boxes/convex hulls over a value-noise terrain
+
+Positions and sizes are drawn in double and rounded to FP32 on store, as the reference's
+Vector3 does.
+"""
+import math
+
+import numpy as np
+
+from . import _abi as A
+from .objects import (Body, Box, ConeTwistConstraint, ConvexPolyhedron, DistanceConstraint,
+                      GSSolver, Heightfield, HingeConstraint, LockConstraint, NaiveBroadphase,
+                      Plane, PointToPointConstraint, Quaternion, SAPBroadphase, Sphere)
+from .world import World
+
+
+def ground_plane():
+    """The demos' ground: Body(mass: 0) + Plane, rotated by setFromEuler(-pi/2, 0, 0)
+    (examples/lib/examples/performance.dart:45-49)."""
+    g = Body(mass=0)
+    g.addShape(Plane())
+    g.quaternion.setFromEuler(-math.pi / 2, 0, 0)
+    return g
+
+
+def shoemake_quaternion(rng):
+    """Uniform random unit quaternion (Shoemake 1992)."""
+    u1, u2, u3 = rng.random(3)
+    a, b = math.sqrt(1 - u1), math.sqrt(u1)
+    return Quaternion(a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2),
+                      b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3))
+
+
+def c1_spheres(seed=1, world=None, **world_kw):
+    rng = np.random.default_rng(seed)
+    w = world or World(gravity=(0, -9.82, 0), broadphase=NaiveBroadphase(),
+                       solver=GSSolver(iterations=10), **world_kw)
+    w.addBody(ground_plane())
+    shape = Sphere(0.5)
+    jitter = rng.uniform(-0.05, 0.05, size=(216, 3))
+    k = 0
+    for iy in range(6):
+        for iz in range(6):
+            for ix in range(6):
+                p = (1.2 * ix + jitter[k, 0], 1.0 + 1.2 * iy + jitter[k, 1], 1.2 * iz + jitter[k, 2])
+                w.addBody(Body(mass=1, position=p, shape=shape))
+                k += 1
+    return w
+
+
+def c2_large_world(seed=2, n=32768, extent=(64.0, 32.0, 64.0), **world_kw):
+    rng = np.random.default_rng(seed)
+    sap = SAPBroadphase()
+    w = World(gravity=(0, -9.82, 0), broadphase=sap, solver=GSSolver(iterations=10), **world_kw)
+    w.addBody(ground_plane())
+    ex, ey, ez = extent
+    for i in range(n):
+        mass = rng.uniform(0.5, 2.0)
+        pos = (rng.uniform(-ex / 2, ex / 2), rng.uniform(1.0, 1.0 + ey), rng.uniform(-ez / 2, ez / 2))
+        if i % 2 == 0:
+            w.addBody(Body(mass=mass, position=pos, shape=Sphere(rng.uniform(0.2, 0.5))))
+        else:
+            he = rng.uniform(0.2, 0.5, size=3)
+            w.addBody(Body(mass=mass, position=pos, quaternion=shoemake_quaternion(rng), shape=Box(he)))
+    sap.autoDetectAxis()
+    return w
+
+
+def c3_world(seed, n=None, kind=None, **world_kw):
+    """One world of the batched box-stack / pile workload."""
+    rng = np.random.default_rng(1_000_003 * 3 + seed)
+    if n is None:
+        n = int(round(math.exp(rng.uniform(math.log(8), math.log(512)))))
+    if kind is None:
+        kind = "stack" if rng.random() < 0.5 else "pile"
+    w = World(gravity=(0, -9.82, 0), broadphase=NaiveBroadphase(), solver=GSSolver(iterations=10),
+              **world_kw)
+    w.addBody(ground_plane())
+    if kind == "stack":
+        height = int(rng.integers(2, 9))
+        cols = (n + height - 1) // height
+        side = int(math.ceil(math.sqrt(cols)))
+        placed = 0
+        for c in range(cols):
+            cx, cz = (c % side) * 1.6, (c // side) * 1.6
+            y = 0.0
+            for _ in range(height):
+                if placed == n:
+                    break
+                he = rng.uniform(0.25, 0.5, size=3)
+                y += he[1] + 0.005
+                w.addBody(Body(mass=1.0, position=(cx + rng.uniform(-0.02, 0.02), y,
+                                                   cz + rng.uniform(-0.02, 0.02)), shape=Box(he)))
+                y += he[1]
+                placed += 1
+    else:
+        side = max(1.0, math.ceil(n ** (1 / 3))) * 1.3
+        for i in range(n):
+            pos = (rng.uniform(0, side), rng.uniform(0.6, 0.6 + side), rng.uniform(0, side))
+            if rng.random() < 0.5:
+                w.addBody(Body(mass=1.0, position=pos, shape=Sphere(rng.uniform(0.2, 0.4))))
+            else:
+                he = rng.uniform(0.25, 0.5, size=3)
+                w.addBody(Body(mass=1.0, position=pos, quaternion=shoemake_quaternion(rng),
+                               shape=Box(he)))
+    return w
+
+
+C4_JOINTS = ("p2p", "hinge", "distance", "conetwist", "lock")
+
+
+def c4_chain_world(seed, length=None, joints=None, **world_kw):
+    """One world of the batched articulated-chain workload: box links (0.1 x 0.3 x 0.1 half
+    extents) stacked along y; each joint drawn from PointToPoint / Hinge(+motor) / Distance /
+    ConeTwist / Lock; collideConnected is set to false on every joint."""
+    rng = np.random.default_rng(1_000_003 * 4 + seed)
+    if length is None:
+        length = int(rng.integers(4, 65))
+    w = World(gravity=(0, -9.82, 0), broadphase=NaiveBroadphase(), solver=GSSolver(iterations=20),
+              **world_kw)
+    w.addBody(ground_plane())
+    shape = Box((0.1, 0.3, 0.1))
+    links = []
+    for i in range(length):
+        p = (rng.uniform(-0.01, 0.01), 0.5 + 0.6 * i, rng.uniform(-0.01, 0.01))
+        b = Body(mass=1.0, position=p, shape=shape)
+        w.addBody(b)
+        links.append(b)
+    for i in range(length - 1):
+        a, b = links[i], links[i + 1]
+        kind = joints[i % len(joints)] if joints else C4_JOINTS[int(rng.integers(0, 5))]
+        if kind == "p2p":
+            c = PointToPointConstraint(a, b, (0, 0.3, 0), (0, -0.3, 0))
+        elif kind == "hinge":
+            c = HingeConstraint(a, b, pivotA=(0, 0.3, 0), pivotB=(0, -0.3, 0), axisA=(1, 0, 0),
+                                axisB=(1, 0, 0))
+            c.enableMotor()
+            c.setMotorSpeed(rng.uniform(-2, 2))
+        elif kind == "distance":
+            c = DistanceConstraint(a, b, 0.6)
+        elif kind == "conetwist":
+            c = ConeTwistConstraint(a, b, pivotA=(0, 0.3, 0), pivotB=(0, -0.3, 0), axisA=(0, 1, 0),
+                                    axisB=(0, 1, 0), angle=rng.uniform(0.2, 0.8), twistAngle=0.0)
+        else:
+            c = LockConstraint(a, b)
+        c.collideConnected = False
+        w.addConstraint(c)
+    # a small push so the column falls over instead of balancing
+    links[-1].velocity.setValues(rng.uniform(-0.5, 0.5), 0, rng.uniform(-0.5, 0.5))
+    return w
+
+
+def value_noise(n, amplitude, rng, octaves=3):
+    """Seeded 2-D value noise on an n x n grid (bilinear, summed octaves), range ~[0, amp]."""
+    out = np.zeros((n, n))
+    total = 0.0
+    for o in range(octaves):
+        cells = 4 * (2 ** o)
+        lattice = rng.random((cells + 2, cells + 2))
+        xs = np.linspace(0, cells, n)
+        i0 = np.floor(xs).astype(int)
+        t = xs - i0
+        t = t * t * (3 - 2 * t)
+        a = lattice[np.ix_(i0, i0)]
+        b = lattice[np.ix_(i0 + 1, i0)]
+        c = lattice[np.ix_(i0, i0 + 1)]
+        d = lattice[np.ix_(i0 + 1, i0 + 1)]
+        tx, ty = t[:, None], t[None, :]
+        layer = (a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty
+        wgt = 0.5 ** o
+        out += wgt * layer
+        total += wgt
+    return amplitude * out / total
+
+
+def quickhull(points):
+    """Convex hull of 3-D points -> (vertices, faces) with faces as CCW vertex-index triangles
+    seen from outside (the winding ConvexPolyhedron.computeNormals expects).  Brute-force
+    plane enumeration over point triples: adequate for the 8-16 point hulls used here."""
+    pts = np.asarray(points, dtype=np.float64)
+    n = len(pts)
+    faces = set()
+    for i in range(n):
+        for j in range(i + 1, n):
+            for k in range(j + 1, n):
+                nrm = np.cross(pts[j] - pts[i], pts[k] - pts[i])
+                ln = np.linalg.norm(nrm)
+                if ln < 1e-9:
+                    continue
+                nrm /= ln
+                d = pts @ nrm - pts[i] @ nrm
+                if np.all(d <= 1e-9):
+                    faces.add((i, j, k))
+                elif np.all(d >= -1e-9):
+                    faces.add((i, k, j))
+    used = sorted({v for f in faces for v in f})
+    remap = {v: t for t, v in enumerate(used)}
+    c = pts[used].mean(axis=0)
+    out = []
+    for f in sorted(faces):
+        a, b, cc = (pts[v] for v in f)
+        nrm = np.cross(b - a, cc - a)
+        if nrm @ (a - c) < 0:
+            f = (f[0], f[2], f[1])
+        out.append([remap[v] for v in f])
+    return pts[used].tolist(), out
+
+
+def random_hull(rng, scale):
+    m = int(rng.integers(8, 17))
+    v = rng.normal(size=(m, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    v *= scale
+    verts, faces = quickhull(v)
+    return ConvexPolyhedron(vertices=verts, faces=faces)
+
+
+def c5_terrain_world(seed, n=None, grid=64, element_size=0.5, amplitude=2.0, **world_kw):
+    """One serving request: N ~ logU[16, 2048] spheres / boxes / convex hulls over a 64x64
+    value-noise heightfield (rotated so that +z of the field points up)."""
+    rng = np.random.default_rng(1_000_003 * 5 + seed)
+    if n is None:
+        n = int(round(math.exp(rng.uniform(math.log(16), math.log(2048)))))
+    w = World(gravity=(0, -9.82, 0), broadphase=NaiveBroadphase(), solver=GSSolver(iterations=10),
+              **world_kw)
+    data = value_noise(grid, amplitude, rng)
+    hf = Heightfield(data, elementSize=element_size)
+    ground = Body(mass=0)
+    ground.addShape(hf)
+    ground.quaternion.setFromEuler(-math.pi / 2, 0, 0)
+    w.addBody(ground)
+    span = (grid - 1) * element_size
+    layers = max(1, int(math.ceil(n / ((span / 1.5) ** 2))))
+    for i in range(n):
+        x = rng.uniform(1.0, span - 1.0)
+        z = -rng.uniform(1.0, span - 1.0)   # field y maps to world -z under the rotation
+        y = amplitude + 1.0 + 1.5 * (i % layers) + rng.uniform(0, 0.5)
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            w.addBody(Body(mass=1.0, position=(x, y, z), shape=Sphere(rng.uniform(0.2, 0.6))))
+        elif kind == 1:
+            w.addBody(Body(mass=1.0, position=(x, y, z), quaternion=shoemake_quaternion(rng),
+                           shape=Box(rng.uniform(0.2, 0.6, size=3))))
+        else:
+            w.addBody(Body(mass=1.0, position=(x, y, z), quaternion=shoemake_quaternion(rng),
+                           shape=random_hull(rng, rng.uniform(0.2, 0.6))))
+    return w

@@ -1,0 +1,246 @@
+"""World: the reference's World API over the device pipeline of libcannon_hip.so.
+
+``World.step(dt)`` is the drop-in for ``World.step(dt)`` (world_class.dart:392-431, fixed
+stepping): the first call compiles the mirrored scene into a ``chp_scene`` recipe, creates the
+device world, and from then on state stays on the device; reading ``body.position`` & co.
+downloads it.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from ._lib import check, load
+from .objects import (GSSolver, NaiveBroadphase, Vector3, _DefaultContactMaterial, _v3,
+                      compile_scene)
+
+
+class Context:
+    """A device context: one HIP stream, an HBM budget and the worlds living on it."""
+
+    def __init__(self, device=0, hbm_budget=0, precision=A.PRECISION_F32):
+        self.lib = load()
+        p = C.c_void_p()
+        rc = self.lib.chp_ctx_create(device, hbm_budget, precision, C.byref(p))
+        if rc != A.CHP_OK:
+            raise RuntimeError("chp_ctx_create failed (%d): no usable gfx950 device?" % rc)
+        self.ptr = p
+        self.precision = precision
+
+    def close(self):
+        if self.ptr:
+            self.lib.chp_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        return check(self.ptr, rc)
+
+    def bytes_in_use(self):
+        return int(self.lib.chp_ctx_bytes_in_use(self.ptr))
+
+    def create_world(self, recipe):
+        wid = C.c_uint32()
+        self.check(self.lib.chp_world_create(self.ptr, recipe.ptr, C.byref(wid)))
+        return int(wid.value)
+
+    def destroy_world(self, wid):
+        self.check(self.lib.chp_world_destroy(self.ptr, wid))
+
+    def num_bodies(self, wid):
+        n = C.c_uint32()
+        self.check(self.lib.chp_world_num_bodies(self.ptr, wid, C.byref(n)))
+        return int(n.value)
+
+    def download(self, wid):
+        n = self.num_bodies(wid)
+        arr = (A.BodyState * max(1, n))()
+        self.check(self.lib.chp_state_download(self.ptr, wid, arr, n))
+        return arr, n
+
+    def upload(self, wid, arr, n):
+        self.check(self.lib.chp_state_upload(self.ptr, wid, arr, n))
+
+    def body_info(self, wid):
+        n = self.num_bodies(wid)
+        arr = (A.BodyInfo * max(1, n))()
+        self.check(self.lib.chp_world_body_info(self.ptr, wid, arr, n))
+        return arr, n
+
+    def step(self, wids, dt, nsteps=1):
+        ids = (C.c_uint32 * len(wids))(*wids)
+        self.check(self.lib.chp_step(self.ptr, ids, len(wids), dt, nsteps))
+
+    def step_async(self, wids, dt, nsteps=1):
+        ids = (C.c_uint32 * len(wids))(*wids)
+        self.check(self.lib.chp_step_async(self.ptr, ids, len(wids), dt, nsteps))
+
+    def sync(self):
+        self.check(self.lib.chp_sync(self.ptr))
+
+    def step_plain(self, wid, dt, nsteps=1):
+        self.check(self.lib.chp_step_plain(self.ptr, wid, dt, nsteps))
+
+    def stats(self, wid):
+        s = A.StepStats()
+        self.check(self.lib.chp_world_stats(self.ptr, wid, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in A.StepStats._fields_}
+
+    def set_capacity(self, wid, max_pairs=0, max_contacts=0, max_rows=0):
+        self.check(self.lib.chp_world_set_capacity(self.ptr, wid, max_pairs, max_contacts, max_rows))
+
+    def broadphase(self, wid, cap=1 << 20):
+        pi = (C.c_int32 * cap)()
+        pj = (C.c_int32 * cap)()
+        n = C.c_uint32()
+        self.check(self.lib.chp_broadphase(self.ptr, wid, pi, pj, cap, C.byref(n)))
+        k = int(n.value)
+        return np.frombuffer(pi, dtype=np.int32, count=k).copy(), np.frombuffer(pj, dtype=np.int32, count=k).copy()
+
+    def step_capture(self, wid, dt, pair_cap=1 << 20, row_cap=1 << 20):
+        pi = (C.c_int32 * pair_cap)()
+        pj = (C.c_int32 * pair_cap)()
+        npairs = C.c_uint32()
+        rows = (A.Row * row_cap)()
+        nrows = C.c_uint32()
+        iters = C.c_int32()
+        self.check(self.lib.chp_step_capture(self.ptr, wid, dt, pi, pj, pair_cap, C.byref(npairs),
+                                             rows, row_cap, C.byref(nrows), C.byref(iters)))
+        k = int(npairs.value)
+        return (np.frombuffer(pi, dtype=np.int32, count=k).copy(),
+                np.frombuffer(pj, dtype=np.int32, count=k).copy(),
+                rows_to_numpy(rows, int(nrows.value)), int(iters.value))
+
+
+_default_ctx = {}
+
+
+def default_context(precision=A.PRECISION_F32):
+    if precision not in _default_ctx:
+        _default_ctx[precision] = Context(precision=precision)
+    return _default_ctx[precision]
+
+
+def state_to_numpy(arr, n):
+    """chp_body_state[] -> dict of float32 arrays."""
+    raw = np.ctypeslib.as_array(arr)[:n] if n else np.zeros(0, dtype=np.dtype(A.BodyState))
+    out = {}
+    for name, _ in A.BodyState._fields_:
+        if name == "reserved":
+            continue
+        out[name] = np.array(raw[name])
+    return out
+
+
+def rows_to_numpy(rows, n):
+    raw = np.ctypeslib.as_array(rows)[:n]
+    return {name: np.array(raw[name]) for name, _ in A.Row._fields_ if name != "reserved"}
+
+
+class _BodyView:
+    """Reads of body state after the world went to the device."""
+
+
+class World:
+    """World({gravity, frictionGravity, allowSleep, broadphase, solver, quatNormalizeFast,
+    quatNormalizeSkip}) (world_class.dart:135-162).  allowSleep must stay false: sleep is not
+    part of the device pipeline."""
+
+    def __init__(self, gravity=None, frictionGravity=None, allowSleep=False, broadphase=None,
+                 solver=None, quatNormalizeFast=False, quatNormalizeSkip=0, context=None,
+                 precision=A.PRECISION_F32):
+        if allowSleep:
+            raise NotImplementedError("allowSleep=True is not supported by the device pipeline")
+        self.gravity = Vector3(*_v3(gravity))
+        self.frictionGravity = None if frictionGravity is None else Vector3(*_v3(frictionGravity))
+        self.broadphase = broadphase if broadphase is not None else NaiveBroadphase()
+        self.solver = solver if solver is not None else GSSolver()
+        self.quatNormalizeFast = quatNormalizeFast
+        self.quatNormalizeSkip = quatNormalizeSkip
+        self.defaultContactMaterial = _DefaultContactMaterial()
+        self.bodies = []
+        self.constraints = []
+        self.contactmaterials = []
+        self.time = 0.0
+        self.stepnumber = 0
+        self.dt = -1.0
+        self._ctx = context
+        self._precision = precision
+        self._wid = None
+        self._state = None
+
+    # ---- construction (mirrors the reference) ----
+    def addBody(self, body):
+        if body in self.bodies:
+            return
+        self._require_host("addBody")
+        body.index = len(self.bodies)
+        body.world = self
+        self.bodies.append(body)
+
+    def addConstraint(self, c):
+        self._require_host("addConstraint")
+        self.constraints.append(c)
+
+    def addContactMaterial(self, cm):
+        self._require_host("addContactMaterial")
+        self.contactmaterials.append(cm)
+
+    def _require_host(self, what):
+        if self._wid is not None:
+            raise NotImplementedError("%s after the first step: rebuild the World" % what)
+
+    # ---- device side ----
+    @property
+    def context(self):
+        if self._ctx is None:
+            self._ctx = default_context(self._precision)
+        return self._ctx
+
+    def recipe(self):
+        return compile_scene(self)
+
+    def _ensure_device(self):
+        if self._wid is None:
+            self._recipe = self.recipe()
+            self._wid = self.context.create_world(self._recipe)
+        return self._wid
+
+    @property
+    def world_id(self):
+        return self._ensure_device()
+
+    def step(self, dt, timeSinceLastCalled=None, maxSubSteps=10):
+        """Fixed stepping (world_class.dart:392-399)."""
+        if timeSinceLastCalled is not None:
+            raise NotImplementedError("interpolated stepping is not part of the device pipeline")
+        wid = self._ensure_device()
+        self.dt = dt
+        self.context.step([wid], dt, 1)
+        self.time += dt
+        self.stepnumber += 1
+        self._state = None
+
+    def step_n(self, dt, n):
+        wid = self._ensure_device()
+        self.dt = dt
+        self.context.step([wid], dt, n)
+        self.time += dt * n
+        self.stepnumber += n
+        self._state = None
+
+    def state(self):
+        """Current per-body state as float32 numpy arrays (downloaded once per step)."""
+        wid = self._ensure_device()
+        if self._state is None:
+            arr, n = self.context.download(wid)
+            self._state = state_to_numpy(arr, n)
+        return self._state
+
+    def stats(self):
+        return self.context.stats(self._ensure_device())

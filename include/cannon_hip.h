@@ -1,0 +1,272 @@
+/*
+ * cannon_hip.h -- C-ABI of libcannon_hip.so, the MI355X (gfx950) implementation of the
+ * per-step rigid-body pipeline of cannon_physics (World.internalStep):
+ *   gravity -> broadphase pairs -> narrowphase contacts + friction -> Gauss-Seidel (SPOOK)
+ *   -> damping -> semi-implicit integration.
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary; every call returns an
+ * int status (CHP_OK or a negative CHP_E_* code) and chp_last_error() explains failures.
+ *
+ * Reference interfaces replaced (paths relative to the cannon_physics repository):
+ *   World.step / World.internalStep        lib/world/world_class.dart:392, :433-701
+ *   Broadphase.collisionPairs (slot)       lib/collision/broadphase.dart:39,
+ *                                          lib/collision/naive_broadphase.dart:14-33,
+ *                                          lib/collision/sap_broadphase.dart:138-166
+ *   Narrowphase.getContacts (slot)         lib/world/narrow_phase.dart:634-721
+ *   Solver.solve / GSSolver.solve (slot)   lib/solver/solver.dart:26, lib/solver/gs_solver.dart:27-133
+ *   Body / Shape / ConvexPolyhedron / Heightfield / Material / ContactMaterial / Constraint
+ *   constructors (scene recipe below)      lib/objects/rigid_body.dart:27-86, lib/rigid_body_shapes/,
+ *                                          lib/material/, lib/constraints/
+ */
+#ifndef CANNON_HIP_H
+#define CANNON_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define CHP_OK 0
+#define CHP_E_INVALID (-1)   /* bad argument, unknown world, malformed scene                  */
+#define CHP_E_OOM (-2)       /* device allocation failed or the context's HBM budget exceeded  */
+#define CHP_E_CAPACITY (-3)  /* a caller buffer or an internal per-world slab is too small     */
+#define CHP_E_HIP (-4)       /* a HIP runtime call failed                                      */
+#define CHP_E_NONFINITE (-5) /* a world produced NaN/Inf state and was quarantined             */
+#define CHP_E_NODEVICE (-6)  /* no usable gfx950 device                                        */
+
+/* ---- enums (numeric values follow the reference's enums) ------------------------------- */
+/* ShapeType index order, lib/rigid_body_shapes/shape.dart:6-18 */
+enum { CHP_SHAPE_SPHERE = 0, CHP_SHAPE_PLANE = 1, CHP_SHAPE_BOX = 2, CHP_SHAPE_CONVEX = 3,
+       CHP_SHAPE_HEIGHTFIELD = 8 };
+/* BodyTypes, lib/objects/rigid_body.dart:15 */
+enum { CHP_BODY_DYNAMIC = 0, CHP_BODY_STATIC = 1, CHP_BODY_KINEMATIC = 2 };
+enum { CHP_BROADPHASE_NAIVE = 0, CHP_BROADPHASE_SAP = 1 };
+enum { CHP_CONSTRAINT_POINT_TO_POINT = 0, CHP_CONSTRAINT_HINGE = 1, CHP_CONSTRAINT_LOCK = 2,
+       CHP_CONSTRAINT_DISTANCE = 3, CHP_CONSTRAINT_CONE_TWIST = 4 };
+/* Equation subclasses, lib/equations/ */
+enum { CHP_ROW_FRICTION = 0, CHP_ROW_CONTACT = 1, CHP_ROW_ROTATIONAL = 2, CHP_ROW_CONE = 3,
+       CHP_ROW_MOTOR = 4 };
+/* arithmetic of the device path: FP32 everywhere, or FP64 arithmetic over FP32-stored vectors
+ * (the reference's own model: vector_math Float32List storage, Dart double expressions) */
+enum { CHP_PRECISION_F32 = 0, CHP_PRECISION_F64 = 1 };
+
+/* ---- scene recipe: what a reference user passes to the constructors --------------------- */
+typedef struct chp_world_desc {          /* World({...}) world_class.dart:135-162 */
+  float gravity[3];
+  int32_t has_friction_gravity;          /* World.frictionGravity != null */
+  float friction_gravity[3];
+  int32_t broadphase;                    /* CHP_BROADPHASE_* */
+  int32_t sap_axis;                      /* SAPBroadphase.axisIndex: 0=x 1=y 2=z */
+  int32_t sap_auto_axis;                 /* call SAPBroadphase.autoDetectAxis() once after build */
+  int32_t iterations;                    /* Solver.iterations (default 10) */
+  double tolerance;                      /* Solver.tolerance (default 1e-7) */
+  int32_t quat_normalize_skip;           /* default 0 */
+  int32_t quat_normalize_fast;           /* default 0 */
+  /* World.defaultContactMaterial, world_class.dart:155-158 + contact_material.dart:37-76 */
+  double friction, restitution;
+  double contact_stiffness, contact_relaxation, friction_stiffness, friction_relaxation;
+} chp_world_desc;
+
+typedef struct chp_material_desc {       /* Material, material.dart:19-25 */
+  double friction, restitution;          /* -1 = unset */
+} chp_material_desc;
+
+typedef struct chp_contact_material_desc { /* ContactMaterial between two materials */
+  int32_t material_a, material_b;
+  double friction, restitution;
+  double contact_stiffness, contact_relaxation, friction_stiffness, friction_relaxation;
+} chp_contact_material_desc;
+
+typedef struct chp_shape_desc {
+  int32_t type;                          /* CHP_SHAPE_* */
+  int32_t collision_response;            /* Shape.collisionResponse (default 1) */
+  int32_t collision_filter_group;        /* Shape default -1, shape.dart:65 */
+  int32_t collision_filter_mask;         /* Shape default -1 */
+  int32_t material;                      /* index into materials, -1 = null */
+  int32_t index;                         /* convex: hull index; heightfield: heightfield index */
+  double radius;                         /* sphere radius */
+  float half_extents[3];                 /* box half extents */
+} chp_shape_desc;
+
+typedef struct chp_hull_desc {           /* ConvexPolyhedron(vertices, faces[, axes]) */
+  uint32_t first_vertex, num_vertices;   /* into hull_vertices (xyz triplets) */
+  uint32_t first_face, num_faces;        /* face f: hull_face_vertex[hull_face_start[f] .. [f+1]) */
+  int32_t has_axes;                      /* uniqueAxes != null (only its null-ness is ever read) */
+} chp_hull_desc;
+
+typedef struct chp_heightfield_desc {    /* Heightfield(data, {minValue,maxValue,elementSize}) */
+  uint32_t nx, ny;                       /* data[nx][ny] */
+  uint32_t first_value;                  /* into heightfield_values, row-major data[i][j] */
+  int32_t has_min_value, has_max_value;
+  double min_value, max_value;
+  double element_size;                   /* an int in the reference; a real here */
+} chp_heightfield_desc;
+
+typedef struct chp_body_shape {          /* Body.addShape(shape, offset, orientation) */
+  uint32_t shape;
+  float offset[3];
+  float orientation[4];                  /* x y z w */
+} chp_body_shape;
+
+typedef struct chp_body_desc {           /* Body({...}) rigid_body.dart:27-86 */
+  double mass;
+  int32_t type;                          /* -1: derived from mass */
+  int32_t collision_filter_group;        /* default 1 */
+  int32_t collision_filter_mask;         /* default -1 */
+  int32_t collision_response;            /* default 1 */
+  int32_t is_trigger;                    /* default 0 */
+  int32_t fixed_rotation;                /* default 0 */
+  int32_t material;                      /* -1 = null */
+  double linear_damping, angular_damping;/* default 0.01 */
+  float position[3], velocity[3];
+  float quaternion[4];                   /* x y z w, given to the constructor */
+  float angular_velocity[3];
+  float linear_factor[3], angular_factor[3];
+  uint32_t first_shape, num_shapes;      /* into body_shapes */
+  int32_t ctor_shape;                    /* 1: body_shapes[first_shape] was the ctor's `shape:` */
+  int32_t set_quaternion;                /* quaternion assigned after construction */
+  float quaternion_after[4];
+  int32_t set_position;                  /* position assigned after construction */
+  float position_after[3];
+} chp_body_desc;
+
+typedef struct chp_constraint_desc {     /* lib/constraints/ constructors */
+  int32_t type;                          /* CHP_CONSTRAINT_* */
+  uint32_t body_a, body_b;
+  int32_t collide_connected;             /* -1: the class default */
+  double max_force;                      /* default 1e6 */
+  float pivot_a[3], pivot_b[3];
+  float axis_a[3], axis_b[3];
+  int32_t has_distance;                  /* DistanceConstraint: explicit distance */
+  double distance;
+  double angle, twist_angle;             /* ConeTwistConstraint */
+  int32_t motor_enabled;                 /* HingeConstraint.enableMotor() */
+  double motor_speed;                    /* HingeConstraint.setMotorSpeed() */
+  int32_t set_motor_max_force;
+  double motor_max_force;
+} chp_constraint_desc;
+
+typedef struct chp_scene {
+  chp_world_desc world;
+  uint32_t num_bodies;            const chp_body_desc* bodies;
+  uint32_t num_body_shapes;       const chp_body_shape* body_shapes;
+  uint32_t num_shapes;            const chp_shape_desc* shapes;
+  uint32_t num_hulls;             const chp_hull_desc* hulls;
+  const float* hull_vertices;     /* 3 floats per vertex */
+  const uint32_t* hull_face_start;/* total faces + 1 entries */
+  const uint32_t* hull_face_vertex;
+  uint32_t num_heightfields;      const chp_heightfield_desc* heightfields;
+  const double* heightfield_values;
+  uint32_t num_materials;         const chp_material_desc* materials;
+  uint32_t num_contact_materials; const chp_contact_material_desc* contact_materials;
+  uint32_t num_constraints;       const chp_constraint_desc* constraints;
+} chp_scene;
+
+/* ---- per-body mutable state (checkpoint / identical-state parity) ---------------------- */
+typedef struct chp_body_state {
+  float position[3];
+  float quaternion[4];
+  float velocity[3];
+  float angular_velocity[3];
+  float force[3];
+  float torque[3];
+  float inv_inertia_world[9];            /* Matrix3 storage order */
+  float aabb_lower[3], aabb_upper[3];
+  int32_t aabb_needs_update;
+  int32_t reserved;
+} chp_body_state;
+
+/* derived per-body constants, for setup parity (Body.updateMassProperties etc.) */
+typedef struct chp_body_info {
+  double mass, inv_mass, bounding_radius;
+  float inertia[3], inv_inertia[3];
+  int32_t type;
+  int32_t reserved;
+} chp_body_info;
+
+/* one solver row (Equation) as the solver sees it, in reference order */
+typedef struct chp_row {
+  int32_t kind;                          /* CHP_ROW_* */
+  int32_t body_i, body_j;                /* world-local body indices (Equation.bi / .bj) */
+  int32_t reserved;
+  float ga_spatial[3], ga_rotational[3]; /* jacobianElementA */
+  float gb_spatial[3], gb_rotational[3]; /* jacobianElementB */
+  double b, inv_c, eps, min_force, max_force, lambda;
+} chp_row;
+
+/* ---- context ----------------------------------------------------------------------------- */
+typedef struct chp_ctx chp_ctx;
+
+/* device: HIP ordinal; hbm_budget: bytes the context may hold on the device (0 = 90% of free);
+ * precision: CHP_PRECISION_*. */
+int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out);
+int chp_ctx_destroy(chp_ctx* ctx);
+const char* chp_last_error(const chp_ctx* ctx);
+/* bytes currently held on the device by this context */
+uint64_t chp_ctx_bytes_in_use(const chp_ctx* ctx);
+
+/* ---- worlds ------------------------------------------------------------------------------ */
+/* Replays the constructors of the recipe on the host (mass properties, AABBs, hull normals and
+ * edges, constraint set-up), uploads the world and returns its id. */
+int chp_world_create(chp_ctx* ctx, const chp_scene* scene, uint32_t* world_id);
+int chp_world_destroy(chp_ctx* ctx, uint32_t world_id);
+int chp_world_num_bodies(chp_ctx* ctx, uint32_t world_id, uint32_t* n);
+int chp_world_body_info(chp_ctx* ctx, uint32_t world_id, chp_body_info* out, uint32_t cap);
+/* capacities of the per-world slabs (pairs, contacts, rows); 0 keeps the current value.
+ * The world keeps its state; a world stopped by CHP_E_CAPACITY can be stepped again. */
+int chp_world_set_capacity(chp_ctx* ctx, uint32_t world_id, uint32_t max_pairs,
+                           uint32_t max_contacts, uint32_t max_rows);
+
+int chp_state_download(chp_ctx* ctx, uint32_t world_id, chp_body_state* out, uint32_t cap);
+int chp_state_upload(chp_ctx* ctx, uint32_t world_id, const chp_body_state* in, uint32_t n);
+
+/* ---- stepping ---------------------------------------------------------------------------- */
+/* Batched path: advances every listed world by nsteps steps of dt in one persistent launch
+ * (one workgroup per world, the whole internalStep fused); state stays on the device.
+ * Equivalent to calling World.step(dt) nsteps times on each world. */
+int chp_step(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, double dt, int32_t nsteps);
+/* Plain path: one world, one launch per stage per step. Same results as chp_step. */
+int chp_step_plain(chp_ctx* ctx, uint32_t world_id, double dt, int32_t nsteps);
+/* Asynchronous batched step on the context's stream; chp_sync waits for it. */
+int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, double dt,
+                   int32_t nsteps);
+int chp_sync(chp_ctx* ctx);
+
+/* Counters of the last step of a world: broadphase pairs, contacts, solver rows, GS
+ * iterations performed, GS dependency levels, status bits (0 = ok) and steps taken so far.
+ * After CHP_E_CAPACITY from a batched step, `steps` tells how far each world got: a world
+ * stops at the step that overflowed, with that step not applied. */
+typedef struct chp_step_stats {
+  uint32_t pairs, contacts, rows, iterations, levels, status, steps, reserved;
+} chp_step_stats;
+int chp_world_stats(chp_ctx* ctx, uint32_t world_id, chp_step_stats* out);
+
+/* ---- slot-level entry points (one world, current device state) --------------------------- */
+/* Broadphase.collisionPairs: pairs in reference emission order (world-local body indices). */
+int chp_broadphase(chp_ctx* ctx, uint32_t world_id, int32_t* pi, int32_t* pj, uint32_t cap,
+                   uint32_t* npairs);
+/* One World.internalStep of dt on the device, also returning what the slots produced:
+ * the broadphase pairs after the collideConnected filter, every solver row in solver order
+ * (friction rows, contact rows, constraint rows) with B, 1/C and the final lambda, and the
+ * number of GS iterations performed (GSSolver.solve's return value). */
+int chp_step_capture(chp_ctx* ctx, uint32_t world_id, double dt,
+                     int32_t* pi, int32_t* pj, uint32_t pair_cap, uint32_t* npairs,
+                     chp_row* rows, uint32_t row_cap, uint32_t* nrows, int32_t* iterations);
+
+/* ---- setup helpers with reference semantics ---------------------------------------------- */
+/* Quaternion.setFromEuler(x, y, z) with order 'xyz' (quaternion.dart:191-232), FP32 result. */
+void chp_quat_from_euler(double x, double y, double z, float out[4]);
+/* Host-only: replays the constructors of a scene recipe (no device needed) and returns the
+ * per-body constants and the initial state the world would start from.  err receives a
+ * message on CHP_E_INVALID. */
+int chp_scene_inspect(const chp_scene* scene, int precision, chp_body_info* info, chp_body_state* state,
+                      uint32_t cap, char* err, uint32_t errcap);
+/* sizeof() of a struct of this header by name (ABI self-check for bindings); 0 if unknown */
+uint32_t chp_abi_struct_size(const char* name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CANNON_HIP_H */

@@ -1,0 +1,114 @@
+/*
+ * ORACLE -- test infrastructure only (see oracle_math.h).
+ * C entry points of the CPU restatement, loaded by tests/ (ctypes) and by bench.py's
+ * cpu_baseline leg.  Built twice: liboracle_f64.so (R = double: the reference's mixed model)
+ * and liboracle_f32.so (R = float).
+ */
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "oracle_world.h"
+
+using namespace orc;
+
+extern "C" {
+
+void* orc_create(const chp_scene* scene) { return build_world(scene); }
+void orc_destroy(void* w) { delete (World*)w; }
+int orc_num_bodies(void* w) { return (int)((World*)w)->bodies.size(); }
+int orc_sap_axis(void* w) { return ((World*)w)->sapAxis; }
+int orc_real_bytes(void) { return (int)sizeof(R); }
+
+void orc_body_info(void* wp, chp_body_info* out) {
+  World& w = *(World*)wp;
+  for (size_t i = 0; i < w.bodies.size(); i++) {
+    const Body& b = *w.bodies[i];
+    chp_body_info& o = out[i];
+    std::memset(&o, 0, sizeof(o));
+    o.mass = (double)b.mass; o.inv_mass = (double)b.invMass; o.bounding_radius = (double)b.boundingRadius;
+    o.inertia[0] = b.inertia.x; o.inertia[1] = b.inertia.y; o.inertia[2] = b.inertia.z;
+    o.inv_inertia[0] = b.invInertia.x; o.inv_inertia[1] = b.invInertia.y; o.inv_inertia[2] = b.invInertia.z;
+    o.type = b.type;
+  }
+}
+
+static void put3(float* d, const V3& v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
+static V3 get3(const float* s) { V3 v; v.x = s[0]; v.y = s[1]; v.z = s[2]; return v; }
+
+void orc_get_state(void* wp, chp_body_state* out) {
+  World& w = *(World*)wp;
+  for (size_t i = 0; i < w.bodies.size(); i++) {
+    const Body& b = *w.bodies[i];
+    chp_body_state& s = out[i];
+    std::memset(&s, 0, sizeof(s));
+    put3(s.position, b.position);
+    s.quaternion[0] = b.quaternion.x; s.quaternion[1] = b.quaternion.y;
+    s.quaternion[2] = b.quaternion.z; s.quaternion[3] = b.quaternion.w;
+    put3(s.velocity, b.velocity); put3(s.angular_velocity, b.angularVelocity);
+    put3(s.force, b.force); put3(s.torque, b.torque);
+    for (int k = 0; k < 9; k++) s.inv_inertia_world[k] = b.invInertiaWorld.e[k];
+    put3(s.aabb_lower, b.aabbLower); put3(s.aabb_upper, b.aabbUpper);
+    s.aabb_needs_update = b.aabbNeedsUpdate ? 1 : 0;
+  }
+}
+
+void orc_set_state(void* wp, const chp_body_state* in) {
+  World& w = *(World*)wp;
+  for (size_t i = 0; i < w.bodies.size(); i++) {
+    Body& b = *w.bodies[i];
+    const chp_body_state& s = in[i];
+    b.position = get3(s.position);
+    b.quaternion.x = s.quaternion[0]; b.quaternion.y = s.quaternion[1];
+    b.quaternion.z = s.quaternion[2]; b.quaternion.w = s.quaternion[3];
+    b.velocity = get3(s.velocity); b.angularVelocity = get3(s.angular_velocity);
+    b.force = get3(s.force); b.torque = get3(s.torque);
+    for (int k = 0; k < 9; k++) b.invInertiaWorld.e[k] = s.inv_inertia_world[k];
+    b.aabbLower = get3(s.aabb_lower); b.aabbUpper = get3(s.aabb_upper);
+    b.aabbNeedsUpdate = s.aabb_needs_update != 0;
+  }
+}
+
+void orc_step(void* wp, double dt, int n) {
+  World& w = *(World*)wp;
+  for (int i = 0; i < n; i++) internal_step(w, (R)dt);
+}
+
+/* steps many worlds, one std::thread per world slice (the "one isolate per world" analogue) */
+void orc_step_many(void** worlds, int nworlds, double dt, int n, int threads) {
+  if (threads < 1) threads = 1;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; t++)
+    pool.emplace_back([=]() {
+      for (int k = t; k < nworlds; k += threads) orc_step(worlds[k], dt, n);
+    });
+  for (auto& th : pool) th.join();
+}
+
+int orc_capture_pairs(void* wp, int* pi, int* pj, int cap) {
+  World& w = *(World*)wp;
+  int n = (int)w.cap.pi.size();
+  for (int k = 0; k < n && k < cap; k++) { pi[k] = w.cap.pi[k]; pj[k] = w.cap.pj[k]; }
+  return n;
+}
+int orc_capture_rows(void* wp, chp_row* rows, int cap) {
+  World& w = *(World*)wp;
+  int n = (int)w.cap.rows.size();
+  for (int k = 0; k < n && k < cap; k++) rows[k] = w.cap.rows[k];
+  return n;
+}
+int orc_capture_iterations(void* wp) { return ((World*)wp)->cap.iterations; }
+int orc_capture_contacts(void* wp) { return ((World*)wp)->cap.contacts; }
+
+/* Broadphase.collisionPairs on the current state (SAP: sorts and refreshes AABBs as the
+ * reference does) */
+int orc_broadphase(void* wp, int* pi, int* pj, int cap) {
+  World& w = *(World*)wp;
+  std::vector<Body*> p1, p2;
+  broadphase_pairs(w, p1, p2);
+  int n = (int)p1.size();
+  for (int k = 0; k < n && k < cap; k++) { pi[k] = p1[k]->index; pj[k] = p2[k]->index; }
+  return n;
+}
+
+}  /* extern "C" */

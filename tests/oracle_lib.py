@@ -1,0 +1,114 @@
+"""ctypes access to the oracle (the CPU restatement under oracle/): test infrastructure."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from cannon_physics_amd import _abi as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "oracle", "_build")
+_libs = {}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def lib(mode="f64"):
+    if mode in _libs:
+        return _libs[mode]
+    path = os.path.join(BUILD, "liboracle_%s.so" % mode)
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "orc_create": (vp, [P(A.Scene)]),
+        "orc_destroy": (None, [vp]),
+        "orc_num_bodies": (C.c_int, [vp]),
+        "orc_sap_axis": (C.c_int, [vp]),
+        "orc_real_bytes": (C.c_int, []),
+        "orc_body_info": (None, [vp, P(A.BodyInfo)]),
+        "orc_get_state": (None, [vp, P(A.BodyState)]),
+        "orc_set_state": (None, [vp, P(A.BodyState)]),
+        "orc_step": (None, [vp, C.c_double, C.c_int]),
+        "orc_step_many": (None, [P(vp), C.c_int, C.c_double, C.c_int, C.c_int]),
+        "orc_capture_pairs": (C.c_int, [vp, P(C.c_int), P(C.c_int), C.c_int]),
+        "orc_capture_rows": (C.c_int, [vp, P(A.Row), C.c_int]),
+        "orc_capture_iterations": (C.c_int, [vp]),
+        "orc_capture_contacts": (C.c_int, [vp]),
+        "orc_broadphase": (C.c_int, [vp, P(C.c_int), P(C.c_int), C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _libs[mode] = L
+    return L
+
+
+class OracleWorld:
+    """One world of the CPU restatement, built from the same chp_scene recipe as the device."""
+
+    def __init__(self, recipe, mode="f64"):
+        self.L = lib(mode)
+        self.recipe = recipe
+        self.ptr = self.L.orc_create(recipe.ptr)
+        self.n = self.L.orc_num_bodies(self.ptr)
+
+    def __del__(self):
+        try:
+            self.L.orc_destroy(self.ptr)
+        except Exception:
+            pass
+
+    def step(self, dt, n=1):
+        self.L.orc_step(self.ptr, dt, n)
+
+    def state_raw(self):
+        arr = (A.BodyState * max(1, self.n))()
+        self.L.orc_get_state(self.ptr, arr)
+        return arr
+
+    def state(self):
+        from cannon_physics_amd.world import state_to_numpy
+        return state_to_numpy(self.state_raw(), self.n)
+
+    def set_state_raw(self, arr):
+        self.L.orc_set_state(self.ptr, arr)
+
+    def body_info(self):
+        arr = (A.BodyInfo * max(1, self.n))()
+        self.L.orc_body_info(self.ptr, arr)
+        raw = np.ctypeslib.as_array(arr)[: self.n]
+        return {k: np.array(raw[k]) for k, _ in A.BodyInfo._fields_ if k != "reserved"}
+
+    def pairs(self, cap=1 << 22):
+        pi = (C.c_int * cap)()
+        pj = (C.c_int * cap)()
+        n = self.L.orc_capture_pairs(self.ptr, pi, pj, cap)
+        return np.array(pi[:n], dtype=np.int32), np.array(pj[:n], dtype=np.int32)
+
+    def broadphase(self, cap=1 << 22):
+        pi = (C.c_int * cap)()
+        pj = (C.c_int * cap)()
+        n = self.L.orc_broadphase(self.ptr, pi, pj, cap)
+        return np.array(pi[:n], dtype=np.int32), np.array(pj[:n], dtype=np.int32)
+
+    def rows(self, cap=1 << 22):
+        from cannon_physics_amd.world import rows_to_numpy
+        arr = (A.Row * cap)()
+        n = self.L.orc_capture_rows(self.ptr, arr, cap)
+        return rows_to_numpy(arr, n)
+
+    def iterations(self):
+        return self.L.orc_capture_iterations(self.ptr)
+
+    def contacts(self):
+        return self.L.orc_capture_contacts(self.ptr)
+
+    def sap_axis(self):
+        return self.L.orc_sap_axis(self.ptr)
