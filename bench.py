@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark of the device pipeline of World.internalStep on one MI355X.
+
+Default workload: the batched box-stack / pile workload -- 4096 independent worlds, body count
+per world log-uniform in [8, 512] (box stacks or piles of boxes and spheres over a plane),
+GSSolver 10 iterations, dt = 1/60, FP32 -- advanced one launch per stage over every world.
+A "step" is one World.step(dt) of every world of the batch.
+
+Prints one JSON line: body-steps/s of the batched path (value), the plain one-world-per-launch
+path on a sample of the same worlds, the roofline of the dominant stage kernel measured with
+HIP events inside the timed region, and the CPU baseline: the C++ restatement of the
+reference semantics (double arithmetic over FP32 storage) on the host cores.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+# algorithmic bytes (per body, pair and row of one step): 13-float state read + written and
+# 6 constants read per dynamic body; two int32 indices per pair; a 64-B row record written by
+# the narrowphase and read by the solver plus a 4-B lambda written back per row
+B_BODY, B_PAIR, B_ROW = 128, 16, 136
+# the GS sweep reads each row record once and writes its lambda once (iterations are meant to
+# stay on chip)
+B_ROW_GS = 68
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_worlds(workload, nworlds, seed0=0):
+    from cannon_physics_amd import scenes
+    recs, dyn, desc = [], [], ""
+    if workload == "c3":
+        for i in range(nworlds):
+            w = scenes.c3_world(seed0 + i)
+            recs.append(w.recipe())
+            dyn.append(len(w.bodies) - 1)
+        desc = ("batched box stacks / piles: %d worlds, log-uniform [8,512] bodies, boxes "
+                "(half-extents U[0.25,0.5]) and spheres (r U[0.2,0.4]) on a plane, NaiveBroadphase, "
+                "GSSolver 10 iterations, dt 1/60" % nworlds)
+        dt = 1 / 60
+    elif workload == "c4":
+        for i in range(nworlds):
+            w = scenes.c4_chain_world(seed0 + i)
+            recs.append(w.recipe())
+            dyn.append(len(w.bodies) - 1)
+        desc = ("batched jointed chains: %d worlds, U{4..64} box links (0.1x0.3x0.1) joined by "
+                "PointToPoint / Hinge+motor / Distance / ConeTwist / Lock, on a plane, GSSolver 20 "
+                "iterations, dt 1/120" % nworlds)
+        dt = 1 / 120
+    elif workload == "c1":
+        w = scenes.c1_spheres()
+        recs.append(w.recipe())
+        dyn.append(216)
+        desc = "216 spheres r=0.5 on a 6x6x6 grid over a plane, NaiveBroadphase, GSSolver 10, dt 1/60"
+        dt = 1 / 60
+    elif workload == "c2":
+        w = scenes.c2_large_world()
+        recs.append(w.recipe())
+        dyn.append(len(w.bodies) - 1)
+        desc = ("one world of 32,768 spheres / rotated boxes in 64x32x64 m over a plane, "
+                "SAPBroadphase (auto axis), GSSolver 10, dt 1/60")
+        dt = 1 / 60
+    else:
+        raise SystemExit("unknown workload " + workload)
+    return recs, dyn, desc, dt
+
+
+def stats_totals(ctx, wids):
+    tot = {"sum_pairs": 0, "sum_rows": 0, "sum_contacts": 0, "steps": 0}
+    for w in wids:
+        s = ctx.stats(w)
+        for k in ("sum_pairs", "sum_rows", "sum_contacts"):
+            tot[k] += s[k]
+        tot["steps"] = max(tot["steps"], s["steps"])
+    return tot
+
+
+def cpu_baseline(recs, dyn, dt, warm, steps, stride):
+    """The oracle (reference semantics, FP64 arithmetic) on every stride-th world, one
+    std::thread per world slice, on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    L = oracle_lib.lib("f64")
+    sample = list(range(0, len(recs), stride))
+    ws = [L.orc_create(recs[i].ptr) for i in sample]
+    arr = (C.c_void_p * len(ws))(*ws)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    L.orc_step_many(arr, len(ws), dt, warm, threads)
+    t0 = time.perf_counter()
+    L.orc_step_many(arr, len(ws), dt, steps, threads)
+    el = time.perf_counter() - t0
+    nb = sum(dyn[i] for i in sample)
+    for w in ws:
+        L.orc_destroy(w)
+    return {"value": nb * steps / el, "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d of the %d worlds (every %d-th), %d dynamic bodies, %d steps after %d warm-up "
+                      "steps, %.1f s wall; C++ restatement of the reference semantics (FP64 "
+                      "arithmetic over FP32 storage), one std::thread per world slice"
+                      % (len(sample), len(recs), stride, nb, steps, warm, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--worlds", type=int, default=None)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--solve-block", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    world_size = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world_size != 1:
+        print("bench.py measures one GPU: worlds shard across GPUs without communication, and "
+              "this build runs and reports the single-GPU figure only (--gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from cannon_physics_amd import _abi as A
+    from cannon_physics_amd.world import Context
+
+    nworlds = args.worlds or {"c3": 4096, "c4": 1024}.get(args.workload, 1)
+    t0 = time.perf_counter()
+    recs, dyn, desc, dt = build_worlds(args.workload, nworlds)
+    log("built %d worlds (%d dynamic bodies) in %.1fs" % (len(recs), sum(dyn), time.perf_counter() - t0))
+
+    prec = A.PRECISION_F64 if args.precision == "f64" else A.PRECISION_F32
+    ctx = Context(device=0, precision=prec)
+    ctx.set_tuning(solve_block=args.solve_block)
+    wids = [ctx.create_world(r) for r in recs]
+    log("uploaded: %.2f GB on the device" % (ctx.bytes_in_use() / 1e9))
+
+    if args.warmup:
+        ctx.step(wids, dt, args.warmup)
+    before = stats_totals(ctx, wids)
+    ctx.set_profiling(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(wids, dt, args.steps)          # synchronous: returns after the device finished
+    wall = time.perf_counter() - t0
+    dev_ms = ctx.last_step_ms()
+    stages = ctx.stage_times()
+    ctx.set_profiling(False)
+    after = stats_totals(ctx, wids)
+    K = args.steps
+    n_dyn = sum(dyn)
+    value = n_dyn * K / wall
+    P = (after["sum_pairs"] - before["sum_pairs"]) / K
+    E = (after["sum_rows"] - before["sum_rows"]) / K
+    Cn = (after["sum_contacts"] - before["sum_contacts"]) / K
+    log("timed %d steps: wall %.3fs device %.1f ms; per step %.0f pairs %.0f contacts %.0f rows"
+        % (K, wall, dev_ms, P, Cn, E))
+    log("stage ms over the timed region: " + json.dumps({k: round(v, 2) for k, v in stages.items()}))
+
+    dom = max(stages, key=stages.get)
+    gs_ms = stages["k_gs"] / K
+    achieved_gs = B_ROW_GS * E / (gs_ms / 1e3) / 1e9
+    b_step = B_BODY * n_dyn + B_PAIR * P + B_ROW * E
+    step_gbs = b_step / (dev_ms / K / 1e3) / 1e9
+
+    plain = None
+    if not args.no_plain and len(wids) > 1:
+        stride = max(1, len(wids) // 32)
+        sample = list(range(0, len(wids), stride))
+        tot_t, tot_bs = 0.0, 0
+        for i in sample:
+            ctx.step_plain(wids[i], dt, 1)
+            t1 = time.perf_counter()
+            ctx.step_plain(wids[i], dt, 5)
+            tot_t += time.perf_counter() - t1
+            tot_bs += dyn[i] * 5
+        plain = {"value": tot_bs / tot_t, "unit": "body-steps/s",
+                 "sample": "%d worlds (every %d-th), 5 steps each, one launch per stage per step "
+                           "and a host synchronisation per step" % (len(sample), stride)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        stride = max(1, len(recs) // 256)
+        cpu = cpu_baseline(recs, dyn, dt, 10, 10 if len(recs) > 1 else 60, stride)
+
+    out = {
+        "metric": "body-steps/s",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": desc, "worlds": len(recs), "dynamic_bodies": n_dyn, "dt": dt,
+                   "pairs_per_step": P, "contacts_per_step": Cn, "rows_per_step": E},
+        "roofline": {"bound": "hbm", "kernel": "k_gs", "achieved": achieved_gs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved_gs / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": B_ROW_GS * E, "avg_launch_ms": gs_ms},
+        "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_gbs / HBM_PEAK_GBS, "bytes_per_step": b_step},
+        "dominant_kernel": dom,
+        "stage_ms_per_step": {k: v / K for k, v in stages.items()},
+        "plain_gpu": plain,
+        "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -231,7 +231,9 @@ class StepStats(C.Structure):
         ("levels", C.c_uint32),
         ("status", C.c_uint32),
         ("steps", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("sum_pairs", C.c_uint32),
+        ("sum_rows", C.c_uint32),
+        ("sum_contacts", C.c_uint32),
     ]
 
 

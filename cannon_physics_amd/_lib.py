@@ -46,6 +46,10 @@ def load():
         "chp_step_plain": (i32, [ctx_p, u32, dbl, i32]),
         "chp_step_async": (i32, [ctx_p, P(u32), u32, dbl, i32]),
         "chp_sync": (i32, [ctx_p]),
+        "chp_last_step_ms": (i32, [ctx_p, P(dbl)]),
+        "chp_ctx_set_tuning": (i32, [ctx_p, C.c_int, C.c_int]),
+        "chp_ctx_set_profiling": (i32, [ctx_p, C.c_int]),
+        "chp_stage_times": (i32, [ctx_p, P(dbl), P(C.c_char_p), u32, P(u32)]),
         "chp_world_stats": (i32, [ctx_p, u32, P(A.StepStats)]),
         "chp_broadphase": (i32, [ctx_p, u32, P(i32), P(i32), u32, P(u32)]),
         "chp_step_capture": (i32, [ctx_p, u32, dbl, P(i32), P(i32), u32, P(u32), P(A.Row), u32,

@@ -20,39 +20,70 @@ using namespace chp;
 // kernels
 // ---------------------------------------------------------------------------------------------
 
-// batched path: one workgroup per listed world, nsteps whole steps per launch
+// One kernel per stage, one workgroup per listed world.  The batched path launches each stage
+// once over every world of the batch; the plain path launches them for one world at a time.
+#define CHP_WORLD_PROLOGUE                                          \
+  const DevWorld<R>& W = worlds[list[blockIdx.x]];                  \
+  if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
+
+// gravity / solve mass / AABBs, broadphase and the collideConnected filter
 template <typename R>
-__global__ __launch_bounds__(CHP_BLOCK) void k_step_fused(const DevWorld<R>* __restrict__ worlds,
-                                                          const int* __restrict__ list, int nsteps, double dt) {
-  __shared__ int sh[CHP_BLOCK];
-  __shared__ R shr[CHP_BLOCK];
-  const DevWorld<R>& W = worlds[list[blockIdx.x]];
-  for (int s = 0; s < nsteps; s++)
-    if (!world_step<R>(W, (R)dt, sh, shr)) break;
+__global__ __launch_bounds__(256) void k_bp(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list) {
+  __shared__ int sh[256];
+  CHP_WORLD_PROLOGUE
+  if (threadIdx.x == 0) W.counters[C_STAGING] = 0;
+  st_prestep(W);
+  __syncthreads();
+  bool ok = W.broadphase == CHP_BROADPHASE_SAP ? st_bp_sap(W, sh) : st_bp_naive(W, sh);
+  if (ok) st_filter(W);
 }
 
-// plain path: one stage of one world per launch
-enum { ST_PRE = 0, ST_BP, ST_FILTER, ST_NARROW, ST_ROWS, ST_LEVELS, ST_GS, ST_POST };
-template <typename R, int STAGE>
-__global__ __launch_bounds__(CHP_BLOCK) void k_stage(const DevWorld<R>* __restrict__ worlds, int wid, double dt) {
-  __shared__ int sh[CHP_BLOCK];
-  __shared__ R shr[CHP_BLOCK];
-  const DevWorld<R>& W = worlds[wid];
-  if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
-  R h = (R)dt;
-  switch (STAGE) {
-    case ST_PRE:
-      if (threadIdx.x == 0) W.counters[C_STAGING] = 0;
-      st_prestep(W);
-      break;
-    case ST_BP: W.broadphase == CHP_BROADPHASE_SAP ? st_bp_sap(W, sh) : st_bp_naive(W, sh); break;
-    case ST_FILTER: st_filter(W); break;
-    case ST_NARROW: st_narrow(W, h); break;
-    case ST_ROWS: st_rows(W, h, sh); break;
-    case ST_LEVELS: st_levels(W); break;
-    case ST_GS: st_gs(W, shr); break;
-    case ST_POST: st_post(W, h); break;
-  }
+// narrowphase: one thread per pair
+template <typename R>
+__global__ __launch_bounds__(64) void k_narrow(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                               double dt) {
+  __shared__ int sh[64];
+  CHP_WORLD_PROLOGUE
+  st_narrow(W, (R)dt);
+}
+
+// equations: friction, contact and constraint rows with B and 1/C
+template <typename R>
+__global__ __launch_bounds__(128) void k_rows(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                              double dt) {
+  __shared__ int sh[128];
+  CHP_WORLD_PROLOGUE
+  st_rows(W, (R)dt, sh);
+}
+
+// GS dependency levels of the step's rows
+template <typename R>
+__global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list) {
+  __shared__ int sh[256];
+  CHP_WORLD_PROLOGUE
+  st_levels(W, sh);
+}
+
+// GS iterations.  Dynamic LDS holds the lambda velocities of worlds with at most lds_bodies
+// bodies; larger worlds use their global scratch.
+template <typename R>
+__global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                            int lds_bodies) {
+  __shared__ int sh[256];
+  __shared__ R shr[256];
+  extern __shared__ float lds[];
+  CHP_WORLD_PROLOGUE
+  bool in_lds = W.nb <= lds_bodies;
+  st_gs(W, shr, in_lds ? lds : W.vl, in_lds ? lds + 3 * lds_bodies : W.wl, W.absdl);
+}
+
+// velocities, damping, integration, clearForces
+template <typename R>
+__global__ __launch_bounds__(256) void k_post(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                              double dt) {
+  __shared__ int sh[256];
+  CHP_WORLD_PROLOGUE
+  st_post(W, (R)dt);
 }
 
 // broadphase only (Broadphase.collisionPairs on the current state)
@@ -135,6 +166,13 @@ struct chp_ctx {
   size_t d_status_cap = 0;
   std::string err;
   std::mutex mu;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the last chp_step / chp_step_plain on the stream
+  bool timed = false;
+  bool profile = false;                      // per-stage events in chp_step
+  std::vector<hipEvent_t> sev;               // 7 events per step
+  std::vector<double> stage_ms;              // accumulated per stage over the last chp_step
+  int solve_block = 256;
+  int lds_cap_bodies = 2048;
 };
 
 static int fail(chp_ctx* c, int code, const std::string& m) {
@@ -230,6 +268,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   SCR(contacts, ContactRec<R>, contact_cap);
   SCR(rows, RowRec<R>, row_cap);
   SCR(level, int, row_cap); SCR(sched, int, row_cap); SCR(lvl_start, int, row_cap + 2);
+  SCR(row_ab, int, 2 * row_cap); SCR(bcnt, int, nb); SCR(blist, int, 2 * row_cap); SCR(absdl, R, row_cap);
   SCR(last_level, int, std::max(nb, row_cap + 1));
   SCR(sort_keys, unsigned long long, np2);
   size_t o_cnt = L.add<int>(C_COUNT);
@@ -373,41 +412,91 @@ static int check_status(chp_ctx* ctx, const uint32_t* ids, uint32_t n) {
   return CHP_OK;
 }
 
+enum { K_BP = 0, K_NARROW, K_ROWS, K_LEVELS, K_GS, K_POST, K_NSTAGE };
+static const char* kStageNames[K_NSTAGE] = {"k_bp", "k_narrow", "k_rows", "k_levels", "k_gs", "k_post"};
+
+// launches the stage kernels of one step for the worlds in d_list[0..n); ev (optional) gets
+// K_NSTAGE + 1 events around them
+template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, int n, int max_nb, double dt,
+                                             hipEvent_t* ev) {
+  const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
+  int lds_bodies = std::min(max_nb, ctx->lds_cap_bodies);
+  size_t lds = (size_t)lds_bodies * 6 * sizeof(float);
+  hipStream_t st = ctx->stream;
+  if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
+  k_bp<R><<<n, 256, 0, st>>>(dw, d_list);
+  if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
+  k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, dt);
+  if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
+  k_rows<R><<<n, 128, 0, st>>>(dw, d_list, dt);
+  if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
+  k_levels<R><<<n, 256, 0, st>>>(dw, d_list);
+  if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
+  k_gs<R><<<n, ctx->solve_block, lds, st>>>(dw, d_list, lds_bodies);
+  if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
+  k_post<R><<<n, 256, 0, st>>>(dw, d_list, dt);
+  if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
+  HIPCHK(ctx, hipGetLastError());
+  return CHP_OK;
+}
+
 template <typename R> static int step_t(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt, int nsteps, bool wait) {
   int r = prepare<R>(ctx, ids, n, dt);
   if (r != CHP_OK) return r;
   if (ctx->d_list_cap < n) {
-    if (ctx->d_list) hipFree(ctx->d_list);
+    if (ctx->d_list) (void)hipFree(ctx->d_list);
     HIPCHK(ctx, hipMalloc(&ctx->d_list, (size_t)n * sizeof(int)));
     ctx->d_list_cap = n;
   }
   // largest worlds first so that the long workgroups start early
   std::vector<int> list(ids, ids + n);
   std::stable_sort(list.begin(), list.end(), [&](int a, int b) { return ctx->worlds[a].nb > ctx->worlds[b].nb; });
+  int max_nb = ctx->worlds[list[0]].nb;
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, list.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  k_step_fused<R><<<n, CHP_BLOCK, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, ctx->d_list, nsteps, dt);
-  HIPCHK(ctx, hipGetLastError());
-  if (!wait) return CHP_OK;
+  size_t nev = ctx->profile ? (size_t)nsteps * (K_NSTAGE + 1) : 0;
+  while (ctx->sev.size() < nev) {
+    hipEvent_t e;
+    HIPCHK(ctx, hipEventCreate(&e));
+    ctx->sev.push_back(e);
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int s = 0; s < nsteps; s++) {
+    r = launch_step<R>(ctx, ctx->d_list, (int)n, max_nb, dt, ctx->profile ? &ctx->sev[s * (K_NSTAGE + 1)] : nullptr);
+    if (r != CHP_OK) return r;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  ctx->timed = true;
+  ctx->stage_ms.assign(K_NSTAGE, 0.0);
+  if (!wait && !ctx->profile) return CHP_OK;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->profile)
+    for (int s = 0; s < nsteps; s++)
+      for (int k = 0; k < K_NSTAGE; k++) {
+        float f = 0;
+        HIPCHK(ctx, hipEventElapsedTime(&f, ctx->sev[s * (K_NSTAGE + 1) + k], ctx->sev[s * (K_NSTAGE + 1) + k + 1]));
+        ctx->stage_ms[k] += f;
+      }
   return check_status(ctx, ids, n);
 }
 
 template <typename R> static int step_plain_t(chp_ctx* ctx, uint32_t id, double dt, int nsteps) {
   int r = prepare<R>(ctx, &id, 1, dt);
   if (r != CHP_OK) return r;
-  const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
-  for (int s = 0; s < nsteps; s++) {
-    k_stage<R, ST_PRE><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
-    k_stage<R, ST_BP><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
-    k_stage<R, ST_FILTER><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
-    k_stage<R, ST_NARROW><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
-    k_stage<R, ST_ROWS><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
-    k_stage<R, ST_LEVELS><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
-    k_stage<R, ST_GS><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
-    k_stage<R, ST_POST><<<1, CHP_BLOCK, 0, ctx->stream>>>(dw, (int)id, dt);
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->d_list_cap < 1) {
+    HIPCHK(ctx, hipMalloc(&ctx->d_list, sizeof(int)));
+    ctx->d_list_cap = 1;
   }
+  int one = (int)id;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int s = 0; s < nsteps; s++) {
+    r = launch_step<R>(ctx, ctx->d_list, 1, ctx->worlds[id].nb, dt, nullptr);
+    if (r != CHP_OK) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the slot-by-slot drop-in hands back each step
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  ctx->timed = true;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return check_status(ctx, &id, 1);
 }
 
@@ -599,6 +688,9 @@ int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out
   hipMemGetInfo(&fr, &tot);
   c->budget = hbm_budget ? hbm_budget : (uint64_t)(0.9 * (double)fr);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CHP_E_HIP; }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return CHP_E_HIP; }
+  if (const char* e = getenv("CHP_SOLVE_BLOCK")) c->solve_block = atoi(e);
+  if (c->solve_block != 64 && c->solve_block != 128 && c->solve_block != 256) c->solve_block = 256;
   *out = c;
   return CHP_OK;
 }
@@ -612,7 +704,10 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   if (ctx->d_list) hipFree(ctx->d_list);
   if (ctx->d_status) hipFree(ctx->d_status);
   if (ctx->d_cptrs) hipFree(ctx->d_cptrs);
-  hipStreamDestroy(ctx->stream);
+  for (hipEvent_t e : ctx->sev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CHP_OK;
 }
@@ -729,6 +824,9 @@ int chp_world_stats(chp_ctx* ctx, uint32_t id, chp_step_stats* out) {
   out->levels = st[C_LEVELS];
   out->status = st[C_STATUS];
   out->steps = st[C_STEP];
+  out->sum_pairs = (uint32_t)st[C_SUM_PAIRS];
+  out->sum_rows = (uint32_t)st[C_SUM_ROWS];
+  out->sum_contacts = (uint32_t)st[C_SUM_CONTACTS];
   return CHP_OK;
 }
 
@@ -770,6 +868,48 @@ int chp_scene_inspect(const chp_scene* scene, int precision, chp_body_info* info
   if (precision == CHP_PRECISION_F64) return inspect_t<double>(scene, info, state, cap, err, errcap);
   if (precision == CHP_PRECISION_F32) return inspect_t<float>(scene, info, state, cap, err, errcap);
   return CHP_E_INVALID;
+}
+
+int chp_last_step_ms(chp_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return CHP_E_INVALID;
+  if (!ctx->timed) return fail(ctx, CHP_E_INVALID, "no step has been timed yet");
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+  float f = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+  *ms = f;
+  return CHP_OK;
+}
+
+int chp_ctx_set_profiling(chp_ctx* ctx, int on) {
+  if (!ctx) return CHP_E_INVALID;
+  ctx->profile = on != 0;
+  return CHP_OK;
+}
+
+int chp_stage_times(chp_ctx* ctx, double* ms, const char** names, uint32_t cap, uint32_t* n) {
+  if (!ctx || !n) return CHP_E_INVALID;
+  *n = K_NSTAGE;
+  if (cap < K_NSTAGE) return fail(ctx, CHP_E_CAPACITY, "need room for every stage");
+  for (int k = 0; k < K_NSTAGE; k++) {
+    if (ms) ms[k] = k < (int)ctx->stage_ms.size() ? ctx->stage_ms[k] : 0.0;
+    if (names) names[k] = kStageNames[k];
+  }
+  return CHP_OK;
+}
+
+int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int lds_cap_bodies) {
+  if (!ctx) return CHP_E_INVALID;
+  if (solve_block != 0) {
+    if (solve_block != 64 && solve_block != 128 && solve_block != 256)
+      return fail(ctx, CHP_E_INVALID, "solve_block must be 64, 128 or 256");
+    ctx->solve_block = solve_block;
+  }
+  if (lds_cap_bodies > 0) {
+    if (lds_cap_bodies > 2560) return fail(ctx, CHP_E_INVALID, "lds_cap_bodies above 2560 exceeds 64 KiB of LDS");
+    ctx->lds_cap_bodies = lds_cap_bodies;
+  }
+  return CHP_OK;
 }
 
 uint32_t chp_abi_struct_size(const char* name) {

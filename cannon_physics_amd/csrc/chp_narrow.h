@@ -306,10 +306,11 @@ template <typename R> struct NPCtx {
     }
   }
 
-  // ConvexPolyhedron.project (convex_polyhedron.dart:843-883)
-  CHP_HD static void project(const HullView<R>& H, f3 axis, f3 pos, f4 q, R& mx, R& mn) {
+  // ConvexPolyhedron.project (convex_polyhedron.dart:843-883).  lo is
+  // Transform.pointToLocalFrame(pos, quat, 0), which depends only on the hull's pose: the
+  // callers compute it once per hull and pass it in (same arithmetic, evaluated once).
+  CHP_HD static void project(const HullView<R>& H, f3 axis, f3 lo, f4 q, R& mx, R& mn) {
     f3 la = vector_to_local<R>(q, axis);
-    f3 lo = point_to_local<R>(pos, q, mk3(0.f, 0.f, 0.f));
     R add = dot3<R>(lo, la);
     R a = dot3<R>(H.v[0], la), b = a;
     for (int i = 1; i < H.nv; i++) {
@@ -323,11 +324,11 @@ template <typename R> struct NPCtx {
     mx = a; mn = b;
   }
   // testSepAxis (:360-384): false when separated
-  CHP_HD static bool testSepAxis(const HullView<R>& A, f3 axis, const HullView<R>& B, f3 pa, f4 qa, f3 pb, f4 qb,
+  CHP_HD static bool testSepAxis(const HullView<R>& A, f3 axis, const HullView<R>& B, f3 loA, f4 qa, f3 loB, f4 qb,
                                  R& depth) {
     R maxA, minA, maxB, minB;
-    project(A, axis, pa, qa, maxA, minA);
-    project(B, axis, pb, qb, maxB, minB);
+    project(A, axis, loA, qa, maxA, minA);
+    project(B, axis, loB, qb, maxB, minB);
     if (maxA < minB || maxB < minA) return false;
     R d0 = maxA - minB, d1 = maxB - minA;
     depth = d0 < d1 ? d0 : d1;
@@ -337,12 +338,14 @@ template <typename R> struct NPCtx {
   CHP_HD static bool findSepAxis(const HullView<R>& A, const HullView<R>& B, f3 pa, f4 qa, f3 pb, f4 qb, bool face0_only,
                                  f3& target) {
     R dmin = (R)__builtin_huge_val();
+    const f3 zero = mk3(0.f, 0.f, 0.f);
+    f3 loA = point_to_local<R>(pa, qa, zero), loB = point_to_local<R>(pb, qb, zero);
     if (A.axes) {
       int nA = face0_only ? 1 : A.nf;
       for (int i = 0; i < nA; i++) {
         f3 n = qrot<R>(qa, A.fn[i]);
         R d;
-        if (!testSepAxis(A, n, B, pa, qa, pb, qb, d)) return false;
+        if (!testSepAxis(A, n, B, loA, qa, loB, qb, d)) return false;
         if (d < dmin) { dmin = d; target = n; }
       }
     }
@@ -350,7 +353,7 @@ template <typename R> struct NPCtx {
       for (int i = 0; i < B.nf; i++) {
         f3 n = qrot<R>(qb, B.fn[i]);
         R d;
-        if (!testSepAxis(A, n, B, pa, qa, pb, qb, d)) return false;
+        if (!testSepAxis(A, n, B, loA, qa, loB, qb, d)) return false;
         if (d < dmin) { dmin = d; target = n; }
       }
     }
@@ -362,7 +365,7 @@ template <typename R> struct NPCtx {
         if (!almost_zero3<R>(cr)) {
           cr = normalize3<R>(cr);
           R d;
-          if (!testSepAxis(A, cr, B, pa, qa, pb, qb, d)) return false;
+          if (!testSepAxis(A, cr, B, loA, qa, loB, qb, d)) return false;
           if (d < dmin) { dmin = d; target = cr; }
         }
       }

@@ -10,7 +10,8 @@ namespace chp {
 #define CHP_KMAX 8
 
 enum { C_STATUS = 0, C_PAIRS = 1, C_CONTACTS = 2, C_ROWS = 3, C_ITERS = 4, C_LEVELS = 5, C_STAGING = 6,
-       C_STEP = 7, C_FROWS = 8, C_CROWS = 9, C_COUNT = 16 };
+       C_STEP = 7, C_FROWS = 8, C_CROWS = 9, C_SUM_PAIRS = 10, C_SUM_ROWS = 11, C_SUM_CONTACTS = 12,
+       C_COUNT = 16 };
 
 __device__ __forceinline__ void ws_fail(int* counters, int bit) { atomicOr(&counters[C_STATUS], bit); }
 
@@ -449,6 +450,9 @@ template <typename R> __device__ bool st_rows(const DevWorld<R>& W, R h, int* sh
     W.counters[C_FROWS] = F;
     W.counters[C_CROWS] = C;
     W.counters[C_CONTACTS] = W.counters[C_STAGING];
+    W.counters[C_SUM_PAIRS] += P;
+    W.counters[C_SUM_ROWS] += E;
+    W.counters[C_SUM_CONTACTS] += W.counters[C_STAGING];
   }
   __syncthreads();
   return true;
@@ -456,70 +460,175 @@ template <typename R> __device__ bool st_rows(const DevWorld<R>& W, R h, int* sh
 
 // ---- Gauss-Seidel -------------------------------------------------------------------------------
 // Exact reordering of GSSolver's sequential sweep: a row only reads and writes the vlambda /
-// wlambda of its two bodies, and only dynamic bodies ever change them, so each row is put on
-// the level after the last earlier row that touches one of its dynamic bodies.  Rows of a
-// level touch disjoint dynamic bodies and run in parallel; every row sees exactly the values
-// it sees in the sequential order.
-template <typename R> __device__ void st_levels(const DevWorld<R>& W) {
+// wlambda of its two bodies, and only dynamic bodies ever change them, so a row may run as
+// soon as every earlier row touching one of its dynamic bodies has run.  The level of a row is
+// its longest-path depth in that dependency graph; rows of one level touch disjoint dynamic
+// bodies and run in parallel, and every row sees exactly the values it sees in the
+// sequential order.
+//
+// Levels are built in rounds: each dynamic body keeps a cursor into its own row list (rows in
+// solver order); a row is ready once it is the cursor row of each of its dynamic bodies; ready
+// rows get the round number and the cursors advance.  The number of rounds is the depth.
+template <typename R> __device__ void st_levels(const DevWorld<R>& W, int* sh) {
   int E = W.counters[C_ROWS];
-  if (threadIdx.x == 0) {
-    int* last = W.last_level;
-    for (int b = 0; b < W.nb; b++) last[b] = 0;
-    int L = 0;
-    for (int r = 0; r < E; r++) {
-      int a = W.rows[r].a, b = W.rows[r].b;
-      bool da = W.type[a] == CHP_BODY_DYNAMIC, db = W.type[b] == CHP_BODY_DYNAMIC;
-      int l = max(da ? last[a] : 0, db ? last[b] : 0);
-      W.level[r] = l;
-      if (da) last[a] = l + 1;
-      if (db) last[b] = l + 1;
-      if (l + 1 > L) L = l + 1;
-    }
-    for (int l = 0; l <= L; l++) W.lvl_start[l] = 0;
-    for (int r = 0; r < E; r++) W.lvl_start[W.level[r] + 1]++;
-    for (int l = 0; l < L; l++) W.lvl_start[l + 1] += W.lvl_start[l];
-    for (int l = 0; l < L; l++) W.last_level[l] = W.lvl_start[l];   // reuse as cursors
-    for (int r = 0; r < E; r++) W.sched[W.last_level[W.level[r]]++] = r;
-    W.counters[C_LEVELS] = L;
+  int nb = W.nb;
+  int* start = W.cnt;     // per body: start of its row list
+  int* head = W.tmp;      // per body: cursor into its list
+  int* bn = W.bcnt;       // per body: list length
+  int* list = W.blist;    // 2 * E entries at most
+  int* lev = W.level;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) start[b] = 0;
+  __syncthreads();
+  for (int r = threadIdx.x; r < E; r += blockDim.x) {
+    int a = W.rows[r].a, b = W.rows[r].b;
+    W.row_ab[2 * r] = a; W.row_ab[2 * r + 1] = b;
+    bool da = W.type[a] == CHP_BODY_DYNAMIC, db = W.type[b] == CHP_BODY_DYNAMIC;
+    lev[r] = (da || db) ? -1 : 0;   // rows without a dynamic body depend on nothing
+    if (da) atomicAdd(&start[a], 1);
+    if (db) atomicAdd(&start[b], 1);
   }
+  __syncthreads();
+  array_exscan(start, nb, sh);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) head[b] = 0;
+  __syncthreads();
+  for (int r = threadIdx.x; r < E; r += blockDim.x) {
+    int a = W.row_ab[2 * r], b = W.row_ab[2 * r + 1];
+    if (W.type[a] == CHP_BODY_DYNAMIC) list[start[a] + atomicAdd(&head[a], 1)] = r;
+    if (W.type[b] == CHP_BODY_DYNAMIC) list[start[b] + atomicAdd(&head[b], 1)] = r;
+  }
+  __syncthreads();
+  // sort each body's list by row index (lists are short) and reset the cursors
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int s0 = start[b], n = head[b];
+    for (int i = 1; i < n; i++) {
+      int v = list[s0 + i], j = i - 1;
+      while (j >= 0 && list[s0 + j] > v) { list[s0 + j + 1] = list[s0 + j]; j--; }
+      list[s0 + j + 1] = v;
+    }
+    bn[b] = n;
+    head[b] = 0;
+  }
+  __syncthreads();
+  int round = 0;
+  for (;;) {
+    bool any = false;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      if (head[b] >= bn[b]) continue;
+      any = true;
+      int r = list[start[b] + head[b]];
+      int a0 = W.row_ab[2 * r], b0 = W.row_ab[2 * r + 1];
+      int other = (a0 == b) ? b0 : a0;
+      bool od = W.type[other] == CHP_BODY_DYNAMIC;
+      bool ready = !od || (head[other] < bn[other] && list[start[other] + head[other]] == r);
+      if (ready && (!od || b < other)) lev[r] = round;   // one writer per row
+    }
+    if (!__syncthreads_or(any)) break;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      if (head[b] < bn[b] && lev[list[start[b] + head[b]]] == round) head[b]++;
+    __syncthreads();
+    round++;
+  }
+  int L = (E > 0 && round == 0) ? 1 : round;
+  // counting sort of the rows by level
+  int* cur = W.last_level;
+  for (int l = threadIdx.x; l <= L; l += blockDim.x) W.lvl_start[l] = 0;
+  __syncthreads();
+  for (int r = threadIdx.x; r < E; r += blockDim.x) atomicAdd(&W.lvl_start[lev[r]], 1);
+  __syncthreads();
+  array_exscan(W.lvl_start, L + 1, sh);
+  for (int l = threadIdx.x; l < L; l += blockDim.x) cur[l] = W.lvl_start[l];
+  __syncthreads();
+  for (int r = threadIdx.x; r < E; r += blockDim.x) W.sched[atomicAdd(&cur[lev[r]], 1)] = r;
+  if (threadIdx.x == 0) W.counters[C_LEVELS] = L;
   __syncthreads();
 }
 
 // GSSolver.solve iterations (gs_solver.dart:76-108) + Equation.computeGWlambda /
-// addToWlambda (equation_class.dart:95-105, 151-169)
-template <typename R> __device__ void st_gs(const DevWorld<R>& W, R* shr) {
+// addToWlambda (equation_class.dart:95-105, 151-169).  vl / wl point to LDS when the world
+// fits, else to the world's global scratch; adl (|dlambda| per row) likewise.  The row a
+// thread handles next is loaded before the current level is processed, so the global-memory
+// latency of the row stream overlaps the level in flight.  |dlambda| is summed in row order by
+// a fixed tree, so the early-exit test does not depend on thread scheduling.
+template <typename R> __device__ __forceinline__ void gs_row(const DevWorld<R>& W, const RowRec<R>& o, int r, R* lam_out,
+                                                             float* vl, float* wl, R* adl) {
+  int a = o.a, b = o.b;
+  f3 va = F3<R>(vl, a), wa = F3<R>(wl, a), vb = F3<R>(vl, b), wb = F3<R>(wl, b);
+  R gwl = mulvec<R>(va, wa, o.sa, o.ra) + mulvec<R>(vb, wb, o.sb, o.rb);
+  R lam = o.lambda;
+  R dl = o.invC * (o.B - gwl - o.eps * lam);
+  if (lam + dl < o.minF) dl = o.minF - lam;
+  else if (lam + dl > o.maxF) dl = o.maxF - lam;
+  *lam_out = lam + dl;
+  adl[r] = dl > (R)0 ? dl : -dl;
+  if (W.type[a] == CHP_BODY_DYNAMIC) {
+    S3<R>(vl, a, axpy3<R>(va, o.ima * dl, o.sa));
+    S3<R>(wl, a, axpy3<R>(wa, dl, o.ia));
+  }
+  if (W.type[b] == CHP_BODY_DYNAMIC) {
+    S3<R>(vl, b, axpy3<R>(vb, o.imb * dl, o.sb));
+    S3<R>(wl, b, axpy3<R>(wb, dl, o.ib));
+  }
+}
+
+template <typename R> __device__ void st_gs(const DevWorld<R>& W, R* shr, float* vl, float* wl, R* adl) {
   int E = W.counters[C_ROWS], L = W.counters[C_LEVELS];
+  int nb = W.nb;
+  if (vl != W.vl) {
+    for (int i = threadIdx.x; i < 3 * nb; i += blockDim.x) { vl[i] = 0.f; wl[i] = 0.f; }
+    __syncthreads();
+  }
   int it = 0;
+  const int* ls = W.lvl_start;
+  const int* sched = W.sched;
   if (E != 0) {
+    // the first row of this thread in level l, if any
+    int l = 0;
+    int k = ls[0] + threadIdx.x;
+    bool have = k < ls[1];
+    RowRec<R> cur;
+    int rcur = have ? sched[k] : 0;
+    if (have) cur = W.rows[rcur];
     for (it = 0; it != W.iterations; it++) {
-      R part = 0;
-      for (int l = 0; l < L; l++) {
-        int k1 = W.lvl_start[l + 1];
-        for (int k = W.lvl_start[l] + threadIdx.x; k < k1; k += blockDim.x) {
-          RowRec<R>& o = W.rows[W.sched[k]];
-          int a = o.a, b = o.b;
-          f3 va = F3<R>(W.vl, a), wa = F3<R>(W.wl, a), vb = F3<R>(W.vl, b), wb = F3<R>(W.wl, b);
-          R gwl = mulvec<R>(va, wa, o.sa, o.ra) + mulvec<R>(vb, wb, o.sb, o.rb);
-          R lam = o.lambda;
-          R dl = o.invC * (o.B - gwl - o.eps * lam);
-          if (lam + dl < o.minF) dl = o.minF - lam;
-          else if (lam + dl > o.maxF) dl = o.maxF - lam;
-          o.lambda = lam + dl;
-          part += dl > (R)0 ? dl : -dl;
-          if (W.type[a] == CHP_BODY_DYNAMIC) {
-            S3<R>(W.vl, a, axpy3<R>(va, o.ima * dl, o.sa));
-            S3<R>(W.wl, a, axpy3<R>(wa, dl, o.ia));
-          }
-          if (W.type[b] == CHP_BODY_DYNAMIC) {
-            S3<R>(W.vl, b, axpy3<R>(vb, o.imb * dl, o.sb));
-            S3<R>(W.wl, b, axpy3<R>(wb, dl, o.ib));
-          }
+      for (l = 0; l < L; l++) {
+        // prefetch: this thread's first row of the next level (wrapping into the next iteration)
+        int ln = (l + 1 < L) ? l + 1 : 0;
+        int kn = ls[ln] + threadIdx.x;
+        bool hn = kn < ls[ln + 1];
+        int rn = hn ? sched[kn] : 0;
+        RowRec<R> nxt;
+        if (hn) nxt = W.rows[rn];
+        if (have) {
+          R lam;
+          gs_row(W, cur, rcur, &lam, vl, wl, adl);
+          W.rows[rcur].lambda = lam;
+        }
+        for (int k2 = ls[l] + threadIdx.x + blockDim.x; k2 < ls[l + 1]; k2 += blockDim.x) {
+          int r2 = sched[k2];
+          RowRec<R> o = W.rows[r2];
+          R lam;
+          gs_row(W, o, r2, &lam, vl, wl, adl);
+          W.rows[r2].lambda = lam;
         }
         __syncthreads();
+        have = hn; rcur = rn;
+        if (hn) {
+          cur = nxt;
+          // lambda of the prefetched row may have been written after the prefetch was issued
+          // (it is the same row when the level holds a single row for this thread and L == 1)
+          if (L == 1) cur.lambda = W.rows[rn].lambda;
+        }
       }
+      // fixed-order sum: each thread sums a contiguous block of rows, then a tree over threads
+      int per = (E + blockDim.x - 1) / blockDim.x;
+      int r0 = threadIdx.x * per, r1 = min(E, r0 + per);
+      R part = 0;
+      for (int r = r0; r < r1; r++) part += adl[r];
       R tot = block_sum<R>(part, shr);
       if (tot * tot < W.tol2) break;
     }
+  }
+  if (vl != W.vl) {  // hand the lambda velocities to st_post
+    for (int i = threadIdx.x; i < 3 * nb; i += blockDim.x) { W.vl[i] = vl[i]; W.wl[i] = wl[i]; }
   }
   if (threadIdx.x == 0) W.counters[C_ITERS] = it;
   __syncthreads();
@@ -593,25 +702,6 @@ template <typename R> __device__ void st_post(const DevWorld<R>& W, R dt) {
   __syncthreads();
   if (threadIdx.x == 0) W.counters[C_STEP] = step + 1;
   __syncthreads();
-}
-
-// one full World.internalStep for the calling workgroup's world; false if the world is in
-// an error state (the step is then not applied)
-template <typename R> __device__ bool world_step(const DevWorld<R>& W, R dt, int* sh, R* shr) {
-  if (threadIdx.x == 0) W.counters[C_STAGING] = 0;
-  if (block_bcast(W.counters[C_STATUS], sh) != 0) return false;
-  st_prestep(W);
-  __syncthreads();
-  bool ok = W.broadphase == CHP_BROADPHASE_SAP ? st_bp_sap(W, sh) : st_bp_naive(W, sh);
-  if (!ok) return false;
-  st_filter(W);
-  st_narrow(W, dt);
-  if (block_bcast(W.counters[C_STATUS], sh) != 0) return false;
-  if (!st_rows(W, dt, sh)) return false;
-  st_levels(W);
-  st_gs(W, shr);
-  st_post(W, dt);
-  return true;
 }
 
 }  // namespace chp

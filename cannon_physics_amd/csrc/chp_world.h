@@ -88,7 +88,8 @@ template <typename R> struct DevWorld {
   int *pc_cnt, *pc_base, *pf_cnt, *pc_off, *pf_off;
   ContactRec<R>* contacts;
   RowRec<R>* rows;
-  int *level, *sched, *lvl_start, *last_level;
+  int *level, *sched, *lvl_start, *last_level, *row_ab, *bcnt, *blist;
+  R* absdl;
   unsigned long long* sort_keys;
   int* counters;   // [0] status [1] pairs [2] contacts [3] rows [4] iterations [5] levels [6] staging
 };

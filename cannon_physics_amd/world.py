@@ -83,6 +83,25 @@ class Context:
     def sync(self):
         self.check(self.lib.chp_sync(self.ptr))
 
+    def last_step_ms(self):
+        ms = C.c_double()
+        self.check(self.lib.chp_last_step_ms(self.ptr, C.byref(ms)))
+        return float(ms.value)
+
+    def set_profiling(self, on=True):
+        self.check(self.lib.chp_ctx_set_profiling(self.ptr, int(on)))
+
+    def stage_times(self):
+        """{kernel name: device ms summed over the last profiled step call}"""
+        ms = (C.c_double * 16)()
+        names = (C.c_char_p * 16)()
+        n = C.c_uint32()
+        self.check(self.lib.chp_stage_times(self.ptr, ms, names, 16, C.byref(n)))
+        return {names[k].decode(): float(ms[k]) for k in range(n.value)}
+
+    def set_tuning(self, solve_block=0, lds_cap_bodies=0):
+        self.check(self.lib.chp_ctx_set_tuning(self.ptr, solve_block, lds_cap_bodies))
+
     def step_plain(self, wid, dt, nsteps=1):
         self.check(self.lib.chp_step_plain(self.ptr, wid, dt, nsteps))
 

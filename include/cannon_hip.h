@@ -233,13 +233,24 @@ int chp_step_plain(chp_ctx* ctx, uint32_t world_id, double dt, int32_t nsteps);
 int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, double dt,
                    int32_t nsteps);
 int chp_sync(chp_ctx* ctx);
+/* device time (HIP events on the context stream) of the last chp_step / chp_step_plain call */
+int chp_last_step_ms(chp_ctx* ctx, double* ms);
+/* launch shape of the GS stage: workgroup size (64, 128 or 256; 0 keeps) and the largest body
+ * count whose lambda velocities are kept in LDS (0 keeps) */
+int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int lds_cap_bodies);
+/* per-stage HIP events in chp_step (costs one event per stage kernel) */
+int chp_ctx_set_profiling(chp_ctx* ctx, int on);
+/* device milliseconds of each stage kernel summed over the last profiled chp_step; names get
+ * the kernel names; *n receives the stage count */
+int chp_stage_times(chp_ctx* ctx, double* ms, const char** names, uint32_t cap, uint32_t* n);
 
 /* Counters of the last step of a world: broadphase pairs, contacts, solver rows, GS
  * iterations performed, GS dependency levels, status bits (0 = ok) and steps taken so far.
  * After CHP_E_CAPACITY from a batched step, `steps` tells how far each world got: a world
  * stops at the step that overflowed, with that step not applied. */
 typedef struct chp_step_stats {
-  uint32_t pairs, contacts, rows, iterations, levels, status, steps, reserved;
+  uint32_t pairs, contacts, rows, iterations, levels, status, steps;
+  uint32_t sum_pairs, sum_rows, sum_contacts;   /* totals over every step taken */
 } chp_step_stats;
 int chp_world_stats(chp_ctx* ctx, uint32_t world_id, chp_step_stats* out);
 

@@ -97,6 +97,19 @@ int orc_capture_rows(void* wp, chp_row* rows, int cap) {
   for (int k = 0; k < n && k < cap; k++) rows[k] = w.cap.rows[k];
   return n;
 }
+/* contacts of the last step in generation order: bi, bj (body indices) and ri, rj, ni (9
+ * floats) per contact */
+int orc_capture_contact_points(void* wp, int* bodies, float* vecs, int cap) {
+  World& w = *(World*)wp;
+  int n = (int)w.contacts.size();
+  for (int k = 0; k < n && k < cap; k++) {
+    const Equation& c = w.contacts[k];
+    bodies[2 * k] = c.bi->index; bodies[2 * k + 1] = c.bj->index;
+    const V3* v[3] = {&c.ri, &c.rj, &c.ni};
+    for (int t = 0; t < 3; t++) { vecs[9 * k + 3 * t] = v[t]->x; vecs[9 * k + 3 * t + 1] = v[t]->y; vecs[9 * k + 3 * t + 2] = v[t]->z; }
+  }
+  return n;
+}
 int orc_capture_iterations(void* wp) { return ((World*)wp)->cap.iterations; }
 int orc_capture_contacts(void* wp) { return ((World*)wp)->cap.contacts; }
 

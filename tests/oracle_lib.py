@@ -41,6 +41,7 @@ def lib(mode="f64"):
         "orc_capture_iterations": (C.c_int, [vp]),
         "orc_capture_contacts": (C.c_int, [vp]),
         "orc_broadphase": (C.c_int, [vp, P(C.c_int), P(C.c_int), C.c_int]),
+        "orc_capture_contact_points": (C.c_int, [vp, P(C.c_int), P(C.c_float), C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -103,6 +104,15 @@ class OracleWorld:
         arr = (A.Row * cap)()
         n = self.L.orc_capture_rows(self.ptr, arr, cap)
         return rows_to_numpy(arr, n)
+
+    def contact_points(self, cap=1 << 20):
+        """contacts of the last step: (bodies (n,2), ri (n,3), rj (n,3), ni (n,3))"""
+        b = (C.c_int * (2 * cap))()
+        v = (C.c_float * (9 * cap))()
+        n = self.L.orc_capture_contact_points(self.ptr, b, v, cap)
+        bb = np.array(b[: 2 * n], dtype=np.int32).reshape(n, 2)
+        vv = np.array(v[: 9 * n], dtype=np.float32).reshape(n, 3, 3)
+        return bb, vv[:, 0], vv[:, 1], vv[:, 2]
 
     def iterations(self):
         return self.L.orc_capture_iterations(self.ptr)

@@ -1,0 +1,65 @@
+"""The library's host-side constructor replay (Body.updateMassProperties with the AABB-box
+inertia, updateBoundingRadius, updateAABB, hull set-up, heightfield pillars, Lock's in-place
+conjugations) against the oracle: bit-identical per-body constants and initial state."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from test_abi import _inspect  # noqa: F401  (shared helper)
+from cannon_physics_amd import _abi as A
+from cannon_physics_amd import scenes
+
+import ctypes as C
+from cannon_physics_amd._lib import load
+from cannon_physics_amd.world import state_to_numpy
+
+SCENES = {
+    "c1_spheres": lambda: scenes.c1_spheres(),
+    "c2_sap_boxes": lambda: scenes.c2_large_world(n=300, extent=(10.0, 6.0, 10.0)),
+    "c3_stack": lambda: scenes.c3_world(11, kind="stack"),
+    "c3_pile": lambda: scenes.c3_world(12, kind="pile"),
+    "c4_chain": lambda: scenes.c4_chain_world(7, length=25),
+    "c5_terrain": lambda: scenes.c5_terrain_world(3, n=24),
+}
+
+
+def lib_inspect(rec, precision):
+    lib = load()
+    n = rec.scene.num_bodies
+    info = (A.BodyInfo * n)()
+    st = (A.BodyState * n)()
+    err = C.create_string_buffer(256)
+    rc = lib.chp_scene_inspect(rec.ptr, precision, info, st, n, err, 256)
+    assert rc == A.CHP_OK, err.value
+    raw = np.ctypeslib.as_array(info)[:n]
+    return {k: np.array(raw[k]) for k, _ in A.BodyInfo._fields_ if k != "reserved"}, state_to_numpy(st, n)
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+@pytest.mark.parametrize("mode", ["f64", "f32"])
+def test_constructor_replay_bit_identical(name, mode):
+    rec = SCENES[name]().recipe()
+    prec = A.PRECISION_F64 if mode == "f64" else A.PRECISION_F32
+    info, st = lib_inspect(rec, prec)
+    ow = O.OracleWorld(rec, mode)
+    oi, os_ = ow.body_info(), ow.state()
+    for k in info:
+        a, b = info[k], oi[k]
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), k
+    for k in os_:
+        assert np.array_equal(st[k].view(np.uint8), os_[k].view(np.uint8)), k
+
+
+def test_sphere_inertia_is_the_aabb_box_inertia():
+    """Body.updateMassProperties approximates every shape by its AABB box
+    (rigid_body.dart:594-601): a sphere of radius r and mass m gets I = m (2r)^2 / 6, not
+    2/5 m r^2."""
+    rec = scenes.c1_spheres().recipe()
+    info, _ = lib_inspect(rec, A.PRECISION_F64)
+    r, m = 0.5, 1.0
+    expect = m * (2 * r) ** 2 / 6
+    assert np.allclose(info["inertia"][1:], expect, rtol=1e-6)
+    assert not np.allclose(info["inertia"][1:], 0.4 * m * r * r)
+    # the static plane: mass 0, infinite AABB -> zero inverse inertia
+    assert info["inv_inertia"][0].tolist() == [0.0, 0.0, 0.0]
+    assert info["type"][0] == A.BODY_STATIC
