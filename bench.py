@@ -75,6 +75,19 @@ def build_worlds(workload, nworlds, seed0=0):
     return recs, dyn, desc, dt
 
 
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    profiles/<kernel>_<workload>_traffic.md (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the
+    same counters on a copy kernel of known size); None when there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_%s_traffic.md" % (kernel, workload, precision))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def stats_totals(ctx, wids):
     tot = {"sum_pairs": 0, "sum_rows": 0, "sum_contacts": 0, "steps": 0}
     for w in wids:
@@ -85,9 +98,10 @@ def stats_totals(ctx, wids):
     return tot
 
 
-def cpu_baseline(recs, dyn, dt, warm, steps, stride):
+def cpu_baseline(recs, dyn, dt, warm, stride, target_s=15.0):
     """The oracle (reference semantics, FP64 arithmetic) on every stride-th world, one
-    std::thread per world slice, on the host cores."""
+    std::thread per world slice, on the host cores; the number of timed steps is chosen so that
+    the timed region takes about target_s seconds."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     L = oracle_lib.lib("f64")
@@ -95,7 +109,10 @@ def cpu_baseline(recs, dyn, dt, warm, steps, stride):
     ws = [L.orc_create(recs[i].ptr) for i in sample]
     arr = (C.c_void_p * len(ws))(*ws)
     threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
     L.orc_step_many(arr, len(ws), dt, warm, threads)
+    per_step = (time.perf_counter() - t0) / max(1, warm)
+    steps = int(max(3, min(600, target_s / max(per_step, 1e-6))))
     t0 = time.perf_counter()
     L.orc_step_many(arr, len(ws), dt, steps, threads)
     el = time.perf_counter() - t0
@@ -117,9 +134,12 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--worlds", type=int, default=None)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--solve-block", type=int, default=64)
+    ap.add_argument("--solve-block", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this size once on the device (the "
+                         "calibration kernel for HBM byte counters)")
     args = ap.parse_args()
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
@@ -186,8 +206,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        stride = max(1, len(recs) // 256)
-        cpu = cpu_baseline(recs, dyn, dt, 10, 10 if len(recs) > 1 else 60, stride)
+        cpu = cpu_baseline(recs, dyn, dt, 5, 1)
 
     out = {
         "metric": "body-steps/s",
@@ -205,7 +224,8 @@ def main():
         "config": {"workload": desc, "worlds": len(recs), "dynamic_bodies": n_dyn, "dt": dt,
                    "pairs_per_step": P, "contacts_per_step": Cn, "rows_per_step": E},
         "roofline": {"bound": "hbm", "kernel": "k_gs", "achieved": achieved_gs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved_gs / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved_gs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, args.precision, "k_gs"),
                      "bytes_per_launch": B_ROW_GS * E, "avg_launch_ms": gs_ms},
         "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes_per_step": b_step},
@@ -216,6 +236,15 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib << 18
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log("copy probe: %d MiB read, %d MiB written" % (args.copy_probe_mib, args.copy_probe_mib))
 
 
 if __name__ == "__main__":

@@ -56,25 +56,84 @@ __global__ __launch_bounds__(128) void k_rows(const DevWorld<R>* __restrict__ wo
   st_rows(W, (R)dt, sh);
 }
 
-// GS dependency levels of the step's rows
-template <typename R>
-__global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list) {
-  __shared__ int sh[256];
-  CHP_WORLD_PROLOGUE
-  st_levels(W, sh);
+// Levels and GS sweeps run one large world per workgroup (the first nbig entries of the list)
+// and one small world (at most SMALL_NB bodies) per wave, blockDim / 64 of them per workgroup.
+constexpr int SMALL_NB = 128;        // bodies of a world run by one wave
+constexpr int BIG_NB_LDS = 512;      // bodies of a large world whose GS / level state fits in LDS
+constexpr int LS_BIG = 1024, LS_SMALL = 256;   // level starts kept in LDS
+
+// dynamic LDS of k_levels / k_gs for a workgroup of bd threads
+static size_t lds_levels(int bd) {
+  size_t big = (size_t)(BIG_NB_LDS + LS_BIG) * sizeof(int), small = (size_t)(SMALL_NB + LS_SMALL) * sizeof(int);
+  return std::max(big, (bd / 64) * small);
+}
+static size_t lds_gs(int bd) {
+  size_t big = (size_t)BIG_NB_LDS * 8 * sizeof(float) + LS_BIG * sizeof(int);
+  size_t small = (size_t)SMALL_NB * 8 * sizeof(float) + LS_SMALL * sizeof(int);
+  return std::max(big, (bd / 64) * small);
 }
 
-// GS iterations.  Dynamic LDS holds the lambda velocities of worlds with at most lds_bodies
-// bodies; larger worlds use their global scratch.
+// the world of this workgroup (large) or of this wave (small); -1: none / quarantined
+template <typename R>
+__device__ __forceinline__ int group_world(const DevWorld<R>* worlds, const int* list, int nbig, int n, int* sh,
+                                           int& wave) {
+  if ((int)blockIdx.x < nbig) {
+    wave = -1;
+    int id = list[blockIdx.x];
+    return block_bcast(worlds[id].counters[C_STATUS], sh) != 0 ? -1 : id;
+  }
+  wave = threadIdx.x >> 6;
+  int idx = nbig + ((int)blockIdx.x - nbig) * (int)(blockDim.x >> 6) + wave;
+  if (idx >= n) return -1;
+  int id = list[idx];
+  return worlds[id].counters[C_STATUS] != 0 ? -1 : id;
+}
+
+// GS dependency levels of the step's rows and the level-ordered GS records
+template <typename R>
+__global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                int nbig, int n) {
+  __shared__ int sh[256];
+  extern __shared__ int lds_i[];
+  int wave;
+  int id = group_world(worlds, list, nbig, n, sh, wave);
+  if (id < 0) return;
+  const DevWorld<R>& W = worlds[id];
+  if (wave < 0) {
+    if (W.nb <= BIG_NB_LDS) st_levels<R, GBlock, true>(W, GBlock(), sh, lds_i, lds_i + BIG_NB_LDS, LS_BIG);
+    else st_levels<R, GBlock, false>(W, GBlock(), sh, W.tmp, nullptr, 0);
+  } else {
+    int* base = lds_i + wave * (SMALL_NB + LS_SMALL);
+    if (W.nb <= SMALL_NB) st_levels<R, GWave, true>(W, GWave(), sh, base, base + SMALL_NB, LS_SMALL);
+    else st_levels<R, GWave, false>(W, GWave(), sh, W.tmp, nullptr, 0);
+  }
+}
+
+// GS iterations
 template <typename R>
 __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                            int lds_bodies) {
+                                            int nbig, int n) {
   __shared__ int sh[256];
-  __shared__ R shr[256];
-  extern __shared__ float lds[];
-  CHP_WORLD_PROLOGUE
-  bool in_lds = W.nb <= lds_bodies;
-  st_gs(W, shr, in_lds ? lds : W.vl, in_lds ? lds + 3 * lds_bodies : W.wl, W.absdl);
+  __shared__ double shd[256];
+  extern __shared__ float4 lds_f4[];
+  int wave;
+  int id = group_world(worlds, list, nbig, n, sh, wave);
+  if (id < 0) return;
+  const DevWorld<R>& W = worlds[id];
+  float* lds = (float*)lds_f4;
+  const int L = W.counters[C_LEVELS];
+  if (wave < 0) {
+    if (W.nb <= BIG_NB_LDS && L + 1 <= LS_BIG)
+      st_gs<R, GBlock, true>(W, GBlock(), shd, sh, lds, (int*)(lds + 8 * BIG_NB_LDS));
+    else
+      st_gs<R, GBlock, false>(W, GBlock(), shd, sh, W.vw, nullptr);
+  } else {
+    float* base = lds + wave * (8 * SMALL_NB + LS_SMALL);
+    if (W.nb <= SMALL_NB && L + 1 <= LS_SMALL)
+      st_gs<R, GWave, true>(W, GWave(), shd, sh, base, (int*)(base + 8 * SMALL_NB));
+    else
+      st_gs<R, GWave, false>(W, GWave(), shd, sh, W.vw, nullptr);
+  }
 }
 
 // velocities, damping, integration, clearForces
@@ -133,6 +192,65 @@ struct Slab {  // one device allocation carved into 256-byte aligned arrays
   }
 };
 
+// Device memory of the worlds: slabs carved first-fit from a few large chunks, so that a batch
+// of thousands of worlds spans few, large allocations (a small address-translation working set)
+// instead of one allocation per world.
+class Arena {
+ public:
+  static constexpr size_t kAlign = 4096;
+  static constexpr size_t kChunk = (size_t)512 << 20;
+  void* alloc(size_t n, hipError_t* err) {
+    *err = hipSuccess;
+    n = (n + kAlign - 1) & ~(kAlign - 1);
+    for (auto& c : chunks_)
+      if (void* p = carve(c, n)) return p;
+    size_t cs = std::max(kChunk, n);
+    void* base = nullptr;
+    *err = hipMalloc(&base, cs);
+    if (*err != hipSuccess) return nullptr;
+    chunks_.push_back(Chunk{(char*)base, cs, {}});
+    chunks_.back().free[0] = cs;
+    return carve(chunks_.back(), n);
+  }
+  void release(void* p, size_t n) {
+    n = (n + kAlign - 1) & ~(kAlign - 1);
+    for (auto& c : chunks_) {
+      if ((char*)p < c.base || (char*)p >= c.base + c.size) continue;
+      size_t off = (char*)p - c.base;
+      auto it = c.free.emplace(off, n).first;
+      auto nx = std::next(it);
+      if (nx != c.free.end() && it->first + it->second == nx->first) { it->second += nx->second; c.free.erase(nx); }
+      if (it != c.free.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) { pv->second += it->second; c.free.erase(it); }
+      }
+      return;
+    }
+  }
+  void destroy() {
+    for (auto& c : chunks_) (void)hipFree(c.base);
+    chunks_.clear();
+  }
+
+ private:
+  struct Chunk {
+    char* base;
+    size_t size;
+    std::map<size_t, size_t> free;  // offset -> length
+  };
+  static void* carve(Chunk& c, size_t n) {
+    for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+      if (it->second < n) continue;
+      size_t off = it->first, len = it->second;
+      c.free.erase(it);
+      if (len > n) c.free[off + n] = len - n;
+      return c.base + off;
+    }
+    return nullptr;
+  }
+  std::vector<Chunk> chunks_;
+};
+
 struct WorldSlot {
   bool live = false;
   int nb = 0;
@@ -155,6 +273,7 @@ struct chp_ctx {
   uint64_t budget = 0;
   uint64_t in_use = 0;
   hipStream_t stream = nullptr;
+  Arena arena;
   std::vector<WorldSlot> worlds;
   void* d_worlds = nullptr;   // DevWorld<R>[capacity]
   size_t d_worlds_cap = 0;
@@ -171,7 +290,7 @@ struct chp_ctx {
   bool profile = false;                      // per-stage events in chp_step
   std::vector<hipEvent_t> sev;               // 7 events per step
   std::vector<double> stage_ms;              // accumulated per stage over the last chp_step
-  int solve_block = 256;
+  int solve_block = 256;   // workgroup of k_levels / k_gs: one large world or blockDim/64 small ones
   int lds_cap_bodies = 2048;
 };
 
@@ -268,7 +387,9 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   SCR(contacts, ContactRec<R>, contact_cap);
   SCR(rows, RowRec<R>, row_cap);
   SCR(level, int, row_cap); SCR(sched, int, row_cap); SCR(lvl_start, int, row_cap + 2);
-  SCR(row_ab, int, 2 * row_cap); SCR(bcnt, int, nb); SCR(blist, int, 2 * row_cap); SCR(absdl, R, row_cap);
+  SCR(row_ab, int, 2 * row_cap); SCR(rpos, int, row_cap); SCR(absdl, R, row_cap + 1); SCR(glam, R, row_cap + 1);
+  fix.emplace_back((void**)&D.gsr, L.add<R>((size_t)4 * GS_GROUPS * row_cap));
+  SCR(vw, float, 8 * nb);
   SCR(last_level, int, std::max(nb, row_cap + 1));
   SCR(sort_keys, unsigned long long, np2);
   size_t o_cnt = L.add<int>(C_COUNT);
@@ -278,14 +399,15 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   if (ctx->in_use + L.size > ctx->budget)
     return fail(ctx, CHP_E_OOM, "world needs " + std::to_string(L.size) + " bytes; context budget " +
                                     std::to_string(ctx->budget) + ", in use " + std::to_string(ctx->in_use));
-  void* mem = nullptr;
-  if (hipMalloc(&mem, L.size) != hipSuccess) return fail(ctx, CHP_E_OOM, "hipMalloc of a world slab failed");
+  hipError_t ae;
+  void* mem = ctx->arena.alloc(L.size, &ae);
+  if (!mem) return fail(ctx, CHP_E_OOM, std::string("device allocation of a world slab failed: ") + hipGetErrorString(ae));
   std::vector<char> host(L.size, 0);
   for (auto& it : L.init) memcpy(host.data() + it.first, it.second.data(), it.second.size());
   hipError_t e = hipMemcpy(mem, host.data(), L.size, hipMemcpyHostToDevice);
-  if (e != hipSuccess) { hipFree(mem); return fail(ctx, CHP_E_HIP, std::string("upload: ") + hipGetErrorString(e)); }
+  if (e != hipSuccess) { ctx->arena.release(mem, L.size); return fail(ctx, CHP_E_HIP, std::string("upload: ") + hipGetErrorString(e)); }
   for (auto& f : fix) *f.first = (char*)mem + f.second;
-  if (S.mem) { hipFree(S.mem); ctx->in_use -= S.bytes; }
+  if (S.mem) { ctx->arena.release(S.mem, S.bytes); ctx->in_use -= S.bytes; }
   S.mem = mem;
   S.bytes = L.size;
   ctx->in_use += L.size;
@@ -342,7 +464,7 @@ template <typename R> static int prepare(chp_ctx* ctx, const uint32_t* ids, uint
   if (ctx->worlds_dirty || ctx->d_worlds_cap < ctx->worlds.size()) {
     size_t cap = std::max<size_t>(ctx->worlds.size(), 1);
     if (ctx->d_worlds_cap < cap) {
-      if (ctx->d_worlds) hipFree(ctx->d_worlds);
+      if (ctx->d_worlds) (void)hipFree(ctx->d_worlds);
       HIPCHK(ctx, hipMalloc(&ctx->d_worlds, cap * sizeof(DevWorld<R>)));
       ctx->d_worlds_cap = cap;
     }
@@ -372,8 +494,8 @@ template <typename R> static int prepare(chp_ctx* ctx, const uint32_t* ids, uint
 
 static int read_status(chp_ctx* ctx, const uint32_t* ids, uint32_t n, std::vector<int>& out) {
   if (ctx->d_status_cap < n) {
-    if (ctx->d_status) hipFree(ctx->d_status);
-    if (ctx->d_cptrs) hipFree(ctx->d_cptrs);
+    if (ctx->d_status) (void)hipFree(ctx->d_status);
+    if (ctx->d_cptrs) (void)hipFree(ctx->d_cptrs);
     HIPCHK(ctx, hipMalloc(&ctx->d_status, (size_t)n * C_COUNT * sizeof(int)));
     HIPCHK(ctx, hipMalloc(&ctx->d_cptrs, (size_t)n * sizeof(int*)));
     ctx->d_status_cap = n;
@@ -417,11 +539,11 @@ static const char* kStageNames[K_NSTAGE] = {"k_bp", "k_narrow", "k_rows", "k_lev
 
 // launches the stage kernels of one step for the worlds in d_list[0..n); ev (optional) gets
 // K_NSTAGE + 1 events around them
-template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, int n, int max_nb, double dt,
+template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, int n, int nbig, double dt,
                                              hipEvent_t* ev) {
   const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
-  int lds_bodies = std::min(max_nb, ctx->lds_cap_bodies);
-  size_t lds = (size_t)lds_bodies * 6 * sizeof(float);
+  const int bd = ctx->solve_block, wpb = bd / 64;
+  const int ngroups = nbig + (n - nbig + wpb - 1) / wpb;
   hipStream_t st = ctx->stream;
   if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
   k_bp<R><<<n, 256, 0, st>>>(dw, d_list);
@@ -430,9 +552,9 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
   k_rows<R><<<n, 128, 0, st>>>(dw, d_list, dt);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
-  k_levels<R><<<n, 256, 0, st>>>(dw, d_list);
+  k_levels<R><<<ngroups, bd, lds_levels(bd), st>>>(dw, d_list, nbig, n);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
-  k_gs<R><<<n, ctx->solve_block, lds, st>>>(dw, d_list, lds_bodies);
+  k_gs<R><<<ngroups, bd, lds_gs(bd), st>>>(dw, d_list, nbig, n);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
   k_post<R><<<n, 256, 0, st>>>(dw, d_list, dt);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
@@ -451,7 +573,8 @@ template <typename R> static int step_t(chp_ctx* ctx, const uint32_t* ids, uint3
   // largest worlds first so that the long workgroups start early
   std::vector<int> list(ids, ids + n);
   std::stable_sort(list.begin(), list.end(), [&](int a, int b) { return ctx->worlds[a].nb > ctx->worlds[b].nb; });
-  int max_nb = ctx->worlds[list[0]].nb;
+  int nbig = 0;
+  while (nbig < (int)n && ctx->worlds[list[nbig]].nb > SMALL_NB) nbig++;
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, list.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   size_t nev = ctx->profile ? (size_t)nsteps * (K_NSTAGE + 1) : 0;
   while (ctx->sev.size() < nev) {
@@ -461,7 +584,7 @@ template <typename R> static int step_t(chp_ctx* ctx, const uint32_t* ids, uint3
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int s = 0; s < nsteps; s++) {
-    r = launch_step<R>(ctx, ctx->d_list, (int)n, max_nb, dt, ctx->profile ? &ctx->sev[s * (K_NSTAGE + 1)] : nullptr);
+    r = launch_step<R>(ctx, ctx->d_list, (int)n, nbig, dt, ctx->profile ? &ctx->sev[s * (K_NSTAGE + 1)] : nullptr);
     if (r != CHP_OK) return r;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -490,7 +613,7 @@ template <typename R> static int step_plain_t(chp_ctx* ctx, uint32_t id, double 
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int s = 0; s < nsteps; s++) {
-    r = launch_step<R>(ctx, ctx->d_list, 1, ctx->worlds[id].nb, dt, nullptr);
+    r = launch_step<R>(ctx, ctx->d_list, 1, ctx->worlds[id].nb > SMALL_NB ? 1 : 0, dt, nullptr);
     if (r != CHP_OK) return r;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the slot-by-slot drop-in hands back each step
   }
@@ -685,7 +808,7 @@ int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out
   c->device = device;
   c->precision = precision;
   size_t fr = 0, tot = 0;
-  hipMemGetInfo(&fr, &tot);
+  (void)hipMemGetInfo(&fr, &tot);
   c->budget = hbm_budget ? hbm_budget : (uint64_t)(0.9 * (double)fr);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CHP_E_HIP; }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return CHP_E_HIP; }
@@ -699,11 +822,11 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   if (!ctx) return CHP_E_INVALID;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  for (auto& w : ctx->worlds) if (w.mem) hipFree(w.mem);
-  if (ctx->d_worlds) hipFree(ctx->d_worlds);
-  if (ctx->d_list) hipFree(ctx->d_list);
-  if (ctx->d_status) hipFree(ctx->d_status);
-  if (ctx->d_cptrs) hipFree(ctx->d_cptrs);
+  ctx->arena.destroy();
+  if (ctx->d_worlds) (void)hipFree(ctx->d_worlds);
+  if (ctx->d_list) (void)hipFree(ctx->d_list);
+  if (ctx->d_status) (void)hipFree(ctx->d_status);
+  if (ctx->d_cptrs) (void)hipFree(ctx->d_cptrs);
   for (hipEvent_t e : ctx->sev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
@@ -730,7 +853,7 @@ int chp_world_destroy(chp_ctx* ctx, uint32_t id) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   WorldSlot& S = ctx->worlds[id];
-  hipFree(S.mem);
+  ctx->arena.release(S.mem, S.bytes);
   ctx->in_use -= S.bytes;
   S = WorldSlot();
   ctx->worlds_dirty = true;

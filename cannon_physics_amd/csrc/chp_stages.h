@@ -57,6 +57,15 @@ template <typename R> __device__ __forceinline__ R block_sum(R v, R* sh) {
   __syncthreads();
   return r;
 }
+// flag bits of the body indices in the solver's row records
+enum : unsigned { GS_DYN = 0x80000000u, GS_NEG = 0x40000000u, GS_IDX = 0x3fffffffu };
+
+// barrier that orders LDS only: outstanding global loads (prefetches) stay in flight
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 // value written by thread 0 of the block, read by all
 __device__ __forceinline__ int block_bcast(int v, int* sh) {
   __syncthreads();
@@ -324,6 +333,8 @@ template <typename R> __device__ void put_row(const DevWorld<R>& W, int r, int k
   o.minF = minF; o.maxF = maxF;
   o.ima = W.ims[a]; o.imb = W.ims[b];
   W.rows[r] = o;
+  W.row_ab[2 * r] = a | (W.type[a] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
+  W.row_ab[2 * r + 1] = b | (W.type[b] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
 }
 // ContactEquation.computeB (contact_equation.dart:34-77)
 template <typename R> __device__ void contact_row(const DevWorld<R>& W, int r, int a, int b, f3 ri, f3 rj, f3 n, R rest,
@@ -466,172 +477,405 @@ template <typename R> __device__ bool st_rows(const DevWorld<R>& W, R h, int* sh
 // bodies and run in parallel, and every row sees exactly the values it sees in the
 // sequential order.
 //
-// Levels are built in rounds: each dynamic body keeps a cursor into its own row list (rows in
-// solver order); a row is ready once it is the cursor row of each of its dynamic bodies; ready
-// rows get the round number and the cursors advance.  The number of rounds is the depth.
-template <typename R> __device__ void st_levels(const DevWorld<R>& W, int* sh) {
-  int E = W.counters[C_ROWS];
-  int nb = W.nb;
-  int* start = W.cnt;     // per body: start of its row list
-  int* head = W.tmp;      // per body: cursor into its list
-  int* bn = W.bcnt;       // per body: list length
-  int* list = W.blist;    // 2 * E entries at most
-  int* lev = W.level;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) start[b] = 0;
-  __syncthreads();
-  for (int r = threadIdx.x; r < E; r += blockDim.x) {
-    int a = W.rows[r].a, b = W.rows[r].b;
-    W.row_ab[2 * r] = a; W.row_ab[2 * r + 1] = b;
-    bool da = W.type[a] == CHP_BODY_DYNAMIC, db = W.type[b] == CHP_BODY_DYNAMIC;
-    lev[r] = (da || db) ? -1 : 0;   // rows without a dynamic body depend on nothing
-    if (da) atomicAdd(&start[a], 1);
-    if (db) atomicAdd(&start[b], 1);
-  }
-  __syncthreads();
-  array_exscan(start, nb, sh);
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) head[b] = 0;
-  __syncthreads();
-  for (int r = threadIdx.x; r < E; r += blockDim.x) {
-    int a = W.row_ab[2 * r], b = W.row_ab[2 * r + 1];
-    if (W.type[a] == CHP_BODY_DYNAMIC) list[start[a] + atomicAdd(&head[a], 1)] = r;
-    if (W.type[b] == CHP_BODY_DYNAMIC) list[start[b] + atomicAdd(&head[b], 1)] = r;
-  }
-  __syncthreads();
-  // sort each body's list by row index (lists are short) and reset the cursors
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    int s0 = start[b], n = head[b];
-    for (int i = 1; i < n; i++) {
-      int v = list[s0 + i], j = i - 1;
-      while (j >= 0 && list[s0 + j] > v) { list[s0 + j + 1] = list[s0 + j]; j--; }
-      list[s0 + j + 1] = v;
+// A world's levels and sweeps run on one group: a whole workgroup for a large world, a single
+// wave for a small one (four small worlds share a workgroup, one per wave, and never meet at a
+// workgroup barrier).
+
+// a whole workgroup on one world
+struct GBlock {
+  int tid, n;
+  __device__ GBlock() : tid(threadIdx.x), n(blockDim.x) {}
+  __device__ void sync() const { __syncthreads(); }
+  __device__ void lds_sync() const { lds_barrier(); }
+  __device__ double sum(double v, double* shd) const {
+    shd[tid] = v;
+    lds_barrier();
+    for (int off = n >> 1; off > 0; off >>= 1) {
+      if (tid < off) shd[tid] = shd[tid] + shd[tid + off];
+      lds_barrier();
     }
-    bn[b] = n;
-    head[b] = 0;
+    double r = shd[0];
+    lds_barrier();
+    return r;
   }
-  __syncthreads();
-  int round = 0;
-  for (;;) {
-    bool any = false;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      if (head[b] >= bn[b]) continue;
-      any = true;
-      int r = list[start[b] + head[b]];
-      int a0 = W.row_ab[2 * r], b0 = W.row_ab[2 * r + 1];
-      int other = (a0 == b) ? b0 : a0;
-      bool od = W.type[other] == CHP_BODY_DYNAMIC;
-      bool ready = !od || (head[other] < bn[other] && list[start[other] + head[other]] == r);
-      if (ready && (!od || b < other)) lev[r] = round;   // one writer per row
-    }
-    if (!__syncthreads_or(any)) break;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-      if (head[b] < bn[b] && lev[list[start[b] + head[b]]] == round) head[b]++;
+  __device__ int imax(int v, int* sh) const {
     __syncthreads();
-    round++;
+    if (tid == 0) sh[0] = 0;
+    __syncthreads();
+    atomicMax(&sh[0], v);
+    __syncthreads();
+    int r = sh[0];
+    __syncthreads();
+    return r;
   }
-  int L = (E > 0 && round == 0) ? 1 : round;
-  // counting sort of the rows by level
-  int* cur = W.last_level;
-  for (int l = threadIdx.x; l <= L; l += blockDim.x) W.lvl_start[l] = 0;
-  __syncthreads();
-  for (int r = threadIdx.x; r < E; r += blockDim.x) atomicAdd(&W.lvl_start[lev[r]], 1);
-  __syncthreads();
-  array_exscan(W.lvl_start, L + 1, sh);
-  for (int l = threadIdx.x; l < L; l += blockDim.x) cur[l] = W.lvl_start[l];
-  __syncthreads();
-  for (int r = threadIdx.x; r < E; r += blockDim.x) W.sched[atomicAdd(&cur[lev[r]], 1)] = r;
-  if (threadIdx.x == 0) W.counters[C_LEVELS] = L;
-  __syncthreads();
+  __device__ int bcast0(int v, int* sh) const {
+    lds_barrier();
+    if (tid == 0) sh[0] = v;
+    lds_barrier();
+    int r = sh[0];
+    lds_barrier();
+    return r;
+  }
+  __device__ int exscan(int* data, int cnt, int* sh) const { return array_exscan(data, cnt, sh); }
+};
+// one wave on one world: LDS and memory accesses of a wave are issued in program order, so a
+// fence that waits for them replaces the barrier
+struct GWave {
+  int tid, n;
+  __device__ GWave() : tid(threadIdx.x & 63), n(64) {}
+  __device__ void sync() const { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+  __device__ void lds_sync() const { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local"); }
+  __device__ double sum(double v, double*) const {
+    // fixed butterfly: every lane ends with the same value
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+  }
+  __device__ int imax(int v, int*) const {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+  }
+  __device__ int bcast0(int v, int*) const { return __shfl(v, 0, 64); }
+  __device__ int exscan(int* data, int cnt, int*) const {
+    int carry = 0;
+    for (int base = 0; base < cnt; base += 64) {
+      int i = base + tid;
+      int v = i < cnt ? data[i] : 0, incl = v;
+      for (int off = 1; off < 64; off <<= 1) {
+        int x = __shfl_up(incl, off, 64);
+        if (tid >= off) incl += x;
+      }
+      if (i < cnt) data[i] = carry + incl - v;
+      carry += __shfl(incl, 63, 64);
+    }
+    sync();
+    return carry;
+  }
+};
+
+// GS record: 6 groups of 4 words, row position k at words [k * 24, k * 24 + 24) -- a wave reads
+// 64 consecutive records
+//   g0 sa.x sa.y sa.z ra.x | g1 ra.y ra.z rb.x rb.y | g2 rb.z ia.x ia.y ia.z
+//   g3 ib.x ib.y ib.z ima  | g4 imb B invC eps      | g5 minF maxF a' b'
+// a' = a | (a dynamic) << 31 | (sb == -sa, else both spatial parts are +0) << 30,
+// b' = b | (b dynamic) << 31.  row_ab holds a and b with the same dynamic bits.
+template <typename R> struct GsWord;
+template <> struct GsWord<float> {
+  typedef unsigned int T;
+  static __device__ __forceinline__ T put(float v) { return __float_as_uint(v); }
+  static __device__ __forceinline__ float get(T w) { return __uint_as_float(w); }
+};
+template <> struct GsWord<double> {
+  typedef unsigned long long T;
+  static __device__ __forceinline__ T put(double v) { return (T)__double_as_longlong(v); }
+  static __device__ __forceinline__ double get(T w) { return __longlong_as_double((long long)w); }
+};
+constexpr int GS_GROUPS = 6;
+
+template <typename R> struct GsRow {
+  f3 sa, ra, rb, ia, ib;
+  R ima, imb, B, invC, eps, minF, maxF;
+  unsigned a, b;
+};
+
+template <typename R> __device__ __forceinline__ void gs_load(const typename GsWord<R>::T* base, int k, GsRow<R>& o) {
+  typedef typename GsWord<R>::T T;
+  T w[4 * GS_GROUPS];
+#pragma unroll
+  for (int g = 0; g < GS_GROUPS; g++) {
+    const T* p = base + ((size_t)k * GS_GROUPS + g) * 4;
+    if constexpr (sizeof(T) == 4) {
+      uint4 v = *(const uint4*)p;
+      w[4 * g] = v.x; w[4 * g + 1] = v.y; w[4 * g + 2] = v.z; w[4 * g + 3] = v.w;
+    } else {
+      ulonglong2 v0 = ((const ulonglong2*)p)[0], v1 = ((const ulonglong2*)p)[1];
+      w[4 * g] = v0.x; w[4 * g + 1] = v0.y; w[4 * g + 2] = v1.x; w[4 * g + 3] = v1.y;
+    }
+  }
+  auto F = [&](int i) { return (float)GsWord<R>::get(w[i]); };
+  auto S = [&](int i) { return GsWord<R>::get(w[i]); };
+  o.sa = mk3(F(0), F(1), F(2)); o.ra = mk3(F(3), F(4), F(5)); o.rb = mk3(F(6), F(7), F(8));
+  o.ia = mk3(F(9), F(10), F(11)); o.ib = mk3(F(12), F(13), F(14));
+  o.ima = S(15); o.imb = S(16); o.B = S(17); o.invC = S(18); o.eps = S(19); o.minF = S(20); o.maxF = S(21);
+  o.a = (unsigned)w[22]; o.b = (unsigned)w[23];
+}
+
+template <typename R> __device__ __forceinline__ void gs_store(typename GsWord<R>::T* base, int k,
+                                                               const RowRec<R>& r, unsigned fa, unsigned fb) {
+  typedef typename GsWord<R>::T T;
+  T w[4 * GS_GROUPS];
+  auto P = [&](int i, R v) { w[i] = GsWord<R>::put(v); };
+  P(0, r.sa.x); P(1, r.sa.y); P(2, r.sa.z); P(3, r.ra.x); P(4, r.ra.y); P(5, r.ra.z);
+  P(6, r.rb.x); P(7, r.rb.y); P(8, r.rb.z); P(9, r.ia.x); P(10, r.ia.y); P(11, r.ia.z);
+  P(12, r.ib.x); P(13, r.ib.y); P(14, r.ib.z); P(15, r.ima); P(16, r.imb); P(17, r.B);
+  P(18, r.invC); P(19, r.eps); P(20, r.minF); P(21, r.maxF);
+  bool neg = !(r.kind == CHP_ROW_ROTATIONAL || r.kind == CHP_ROW_CONE || r.kind == CHP_ROW_MOTOR);
+  w[22] = (T)(fa | (neg ? GS_NEG : 0u));
+  w[23] = (T)fb;
+#pragma unroll
+  for (int g = 0; g < GS_GROUPS; g++) {
+    T* p = base + ((size_t)k * GS_GROUPS + g) * 4;
+    if constexpr (sizeof(T) == 4) {
+      *(uint4*)p = make_uint4(w[4 * g], w[4 * g + 1], w[4 * g + 2], w[4 * g + 3]);
+    } else {
+      ((ulonglong2*)p)[0] = make_ulonglong2(w[4 * g], w[4 * g + 1]);
+      ((ulonglong2*)p)[1] = make_ulonglong2(w[4 * g + 2], w[4 * g + 3]);
+    }
+  }
+}
+
+// Levels.  One wave walks the rows in solver order, 64 at a time: a row starts one level above
+// the last level of each of its dynamic bodies (last[], from the earlier chunks); inside the
+// chunk lane j's level is final once lanes 0..j-1 are done, so a uniform pass over j hands it
+// to the later lanes sharing a body.  The rows are then counting-sorted by level and gathered
+// into the GS records in level order.
+//   last: nb ints (LDS when LDS, else the world's global scratch, read with atomics so that
+//         each chunk sees the previous chunk's values)
+//   hist: hist_cap ints of LDS for the level histogram (global scratch when L + 1 > hist_cap
+//         or not LDS)
+template <typename R, typename G, bool LDS>
+__device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int* sh, int* last, int* hist, int hist_cap) {
+  const int E = W.counters[C_ROWS], nb = W.nb;
+  const int tid = g.tid, bd = g.n;
+  for (int b = tid; b < nb; b += bd) last[b] = -1;
+  g.sync();
+  int lmax = 0;
+  if (tid < 64) {
+    const int lane = tid;
+    int2 ab = make_int2(0, 0);
+    if (lane < E) ab = ((const int2*)W.row_ab)[lane];
+    for (int c0 = 0; c0 < E; c0 += 64) {
+      int r = c0 + lane;
+      bool act = r < E;
+      int2 nab = make_int2(0, 0);   // next chunk, loaded ahead
+      if (r + 64 < E) nab = ((const int2*)W.row_ab)[r + 64];
+      const bool da = act && (ab.x & GS_DYN), db = act && (ab.y & GS_DYN);
+      const int a = ab.x & GS_IDX, b = ab.y & GS_IDX;
+      const int ka = da ? a : -1 - 2 * lane, kb = db ? b : -2 - 2 * lane;   // non-dynamic keys never match
+      int lev = 0;
+      if constexpr (LDS) {
+        if (da) lev = max(lev, last[a] + 1);
+        if (db) lev = max(lev, last[b] + 1);
+      } else {
+        if (da) lev = max(lev, atomicAdd(&last[a], 0) + 1);
+        if (db) lev = max(lev, atomicAdd(&last[b], 0) + 1);
+      }
+      const int n = min(64, E - c0);
+      for (int j = 0; j < n - 1; j++) {
+        int kaj = __builtin_amdgcn_readlane(ka, j), kbj = __builtin_amdgcn_readlane(kb, j);
+        int lj = __builtin_amdgcn_readlane(lev, j);
+        if (lane > j && (ka == kaj || ka == kbj || kb == kaj || kb == kbj)) lev = max(lev, lj + 1);
+      }
+      // the chain through a body has increasing levels: the chunk's last row on it wins
+      if (da) atomicMax(&last[a], lev);
+      if (db) atomicMax(&last[b], lev);
+      if (act) { W.level[r] = lev; lmax = max(lmax, lev + 1); }
+      ab = nab;
+    }
+  }
+  const int L = g.imax(lmax, sh);
+  // counting sort by level; the order inside a level is free (its rows are independent)
+  auto sort = [&](int* cnt) {
+    for (int l = tid; l <= L; l += bd) cnt[l] = 0;
+    g.sync();
+    for (int r = tid; r < E; r += bd) atomicAdd(&cnt[W.level[r]], 1);
+    g.sync();
+    g.exscan(cnt, L + 1, sh);
+    for (int l = tid; l <= L; l += bd) W.lvl_start[l] = cnt[l];
+    g.sync();
+    for (int r = tid; r < E; r += bd) {
+      int k = atomicAdd(&cnt[W.level[r]], 1);
+      W.sched[k] = r;
+      W.rpos[r] = k;
+    }
+    g.sync();
+  };
+  if (LDS && L + 1 <= hist_cap) sort(hist);
+  else sort(W.last_level);
+  typedef typename GsWord<R>::T T;
+  T* gsr = (T*)W.gsr;
+  for (int k = tid; k < E; k += bd) {
+    int r = W.sched[k];
+    RowRec<R> o = W.rows[r];
+    int2 ab = ((const int2*)W.row_ab)[r];
+    gs_store<R>(gsr, k, o, (unsigned)ab.x, (unsigned)ab.y);
+    W.glam[k] = (R)0;
+  }
+  if (tid == 0) W.counters[C_LEVELS] = L;
+  g.sync();
 }
 
 // GSSolver.solve iterations (gs_solver.dart:76-108) + Equation.computeGWlambda /
-// addToWlambda (equation_class.dart:95-105, 151-169).  vl / wl point to LDS when the world
-// fits, else to the world's global scratch; adl (|dlambda| per row) likewise.  The row a
-// thread handles next is loaded before the current level is processed, so the global-memory
-// latency of the row stream overlaps the level in flight.  |dlambda| is summed in row order by
-// a fixed tree, so the early-exit test does not depend on thread scheduling.
-template <typename R> __device__ __forceinline__ void gs_row(const DevWorld<R>& W, const RowRec<R>& o, int r, R* lam_out,
-                                                             float* vl, float* wl, R* adl) {
-  int a = o.a, b = o.b;
-  f3 va = F3<R>(vl, a), wa = F3<R>(wl, a), vb = F3<R>(vl, b), wb = F3<R>(wl, b);
-  R gwl = mulvec<R>(va, wa, o.sa, o.ra) + mulvec<R>(vb, wb, o.sb, o.rb);
-  R lam = o.lambda;
-  R dl = o.invC * (o.B - gwl - o.eps * lam);
-  if (lam + dl < o.minF) dl = o.minF - lam;
-  else if (lam + dl > o.maxF) dl = o.maxF - lam;
-  *lam_out = lam + dl;
-  adl[r] = dl > (R)0 ? dl : -dl;
-  if (W.type[a] == CHP_BODY_DYNAMIC) {
-    S3<R>(vl, a, axpy3<R>(va, o.ima * dl, o.sa));
-    S3<R>(wl, a, axpy3<R>(wa, dl, o.ia));
-  }
-  if (W.type[b] == CHP_BODY_DYNAMIC) {
-    S3<R>(vl, b, axpy3<R>(vb, o.imb * dl, o.sb));
-    S3<R>(wl, b, axpy3<R>(wb, dl, o.ib));
-  }
+// addToWlambda (equation_class.dart:95-105, 151-169) over the level schedule.
+//
+// Early exit: the reference sums |dlambda| in row order and stops once (sum)^2 < tol^2.  Here
+// every thread adds its rows' |dlambda| in double and the group adds the partials; the rounding
+// error of the sequential R sum is at most gamma_{E-1} * sum (all terms are >= 0), so the
+// comparison is decided from the double sum whenever the interval around it does not straddle
+// the threshold.  Otherwise one thread repeats the sequential R sum in solver order from the
+// stored |dlambda|.  Either way the decision is the reference's, bit for bit.
+template <typename R> __device__ __forceinline__ int gs_decide(double D, int n, R tol2) {
+  const double u = sizeof(R) == 4 ? 0x1p-24 : 0x1p-53, ud = 0x1p-53;
+  double g = (double)n * u / (1.0 - (double)n * u), gd = (double)n * ud / (1.0 - (double)n * ud);
+  double slack = 2.0 * (g + gd) + 1e-15;
+  double hi = D * (1.0 + slack), lo = D * (1.0 - slack);
+  R hiR, loR;
+  if constexpr (sizeof(R) == 4) { hiR = __double2float_ru(hi); loR = __double2float_rd(lo); }
+  else { hiR = hi; loR = lo; }
+  if (hiR * hiR < tol2) return 1;    // exit whatever the rounding of the sequential sum
+  if (loR * loR >= tol2) return 0;   // continue whatever it is
+  return -1;                         // undecided, or NaN: the sequential sum decides
 }
 
-template <typename R> __device__ void st_gs(const DevWorld<R>& W, R* shr, float* vl, float* wl, R* adl) {
-  int E = W.counters[C_ROWS], L = W.counters[C_LEVELS];
-  int nb = W.nb;
-  if (vl != W.vl) {
-    for (int i = threadIdx.x; i < 3 * nb; i += blockDim.x) { vl[i] = 0.f; wl[i] = 0.f; }
-    __syncthreads();
+// One row of the sweep: computeGWlambda, the clamped dlambda, addToWlambda.  Returns the new
+// lambda and adds |dlambda| to part.  An idle lane (v false) computes on a valid row but writes
+// no body, stores to the slot ks (a dummy slot) and adds nothing: the memory operations of the
+// step are the same on every lane, which keeps the prefetch counts exact.
+template <typename R>
+__device__ __forceinline__ R gs_row(const GsRow<R>& cr, R clam, bool v, int ks, float* vw, R* lam, R* adl, double& part) {
+  const int a = cr.a & GS_IDX, b = cr.b & GS_IDX;
+  const bool da = v && (cr.a & GS_DYN), db = v && (cr.b & GS_DYN);
+  const f3 z = mk3(0.f, 0.f, 0.f);
+  const f3 sb = (cr.a & GS_NEG) ? neg3(cr.sa) : z;
+  float4 va4 = ((const float4*)vw)[2 * a], wa4 = ((const float4*)vw)[2 * a + 1];
+  float4 vb4 = ((const float4*)vw)[2 * b], wb4 = ((const float4*)vw)[2 * b + 1];
+  f3 va = mk3(va4.x, va4.y, va4.z), wa = mk3(wa4.x, wa4.y, wa4.z);
+  f3 vb = mk3(vb4.x, vb4.y, vb4.z), wb = mk3(wb4.x, wb4.y, wb4.z);
+  R gwl = mulvec<R>(va, wa, cr.sa, cr.ra) + mulvec<R>(vb, wb, sb, cr.rb);
+  R dl = cr.invC * (cr.B - gwl - cr.eps * clam);
+  if (clam + dl < cr.minF) dl = cr.minF - clam;
+  else if (clam + dl > cr.maxF) dl = cr.maxF - clam;
+  R lnew = clam + dl;
+  R adlv = dl > (R)0 ? dl : -dl;
+  lam[ks] = lnew;
+  adl[ks] = adlv;
+  part += v ? (double)adlv : 0.0;
+  if (da) {
+    va = axpy3<R>(va, cr.ima * dl, cr.sa);
+    wa = axpy3<R>(wa, dl, cr.ia);
+    ((float4*)vw)[2 * a] = make_float4(va.x, va.y, va.z, 0.f);
+    ((float4*)vw)[2 * a + 1] = make_float4(wa.x, wa.y, wa.z, 0.f);
   }
-  int it = 0;
+  if (db) {
+    if (a == b) { vb = va; wb = wa; }
+    vb = axpy3<R>(vb, cr.imb * dl, sb);
+    wb = axpy3<R>(wb, dl, cr.ib);
+    ((float4*)vw)[2 * b] = make_float4(vb.x, vb.y, vb.z, 0.f);
+    ((float4*)vw)[2 * b + 1] = make_float4(wb.x, wb.y, wb.z, 0.f);
+  }
+  return lnew;
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// vw: 8 floats per body (vlambda xyz, pad, wlambda xyz, pad).  LDS: vw and ls_lds (L + 1 ints)
+// are LDS arrays; else vw is the world's global scratch and the level starts are read from
+// global memory.  The two forms are separate instantiations so that LDS accesses compile to LDS
+// instructions.
+//
+// The sweep is a sequence of steps, one level of one iteration each.  A thread's first row of
+// each level (position ls[l] + tid) is loaded D steps ahead into a ring of registers, so the
+// row stream's memory latency overlaps D levels; further rows of a level wider than the group
+// are loaded when they are reached.  A row recurs after L steps: when L < D a ring slot may hold
+// a row whose lambda changes before the slot is used, and the new lambda is forwarded to it.
+// lam / adl have one dummy slot at row_cap for idle lanes.
+template <typename R, typename G, bool LDS>
+__device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* shd, int* sh, float* vw, int* ls_lds) {
+  constexpr int D = sizeof(R) == 4 ? 4 : 2;
+  const int E = uni(W.counters[C_ROWS]), L = uni(W.counters[C_LEVELS]), iters = uni(W.iterations);
+  const int nb = W.nb, tid = g.tid, bd = g.n, dummy = W.row_cap;
+  const R tol2 = W.tol2;
   const int* ls = W.lvl_start;
-  const int* sched = W.sched;
-  if (E != 0) {
-    // the first row of this thread in level l, if any
+  if constexpr (LDS) {
+    for (int l = tid; l <= L; l += bd) ls_lds[l] = W.lvl_start[l];
+    ls = ls_lds;
+  }
+  for (int i = tid; i < 8 * nb; i += bd) vw[i] = 0.f;
+  g.sync();
+  int it = 0;
+  typedef typename GsWord<R>::T T;
+  const T* gsr = (const T*)W.gsr;
+  R* lam = W.glam;
+  R* adl = W.absdl;
+  if (E != 0 && iters > 0) {
+    GsRow<R> q[D];
+    R ql[D];
+    int qk[D];
+    bool qv[D];
+    int pl = 0;   // level of the next step to load
+#pragma unroll
+    for (int s = 0; s < D; s++) {
+      int k = ls[pl] + tid;
+      qv[s] = k < ls[pl + 1];
+      qk[s] = qv[s] ? k : 0;
+      gs_load<R>(gsr, qk[s], q[s]);
+      ql[s] = lam[qk[s]];
+      pl = (pl + 1 == L) ? 0 : pl + 1;
+    }
     int l = 0;
-    int k = ls[0] + threadIdx.x;
-    bool have = k < ls[1];
-    RowRec<R> cur;
-    int rcur = have ? sched[k] : 0;
-    if (have) cur = W.rows[rcur];
-    for (it = 0; it != W.iterations; it++) {
-      for (l = 0; l < L; l++) {
-        // prefetch: this thread's first row of the next level (wrapping into the next iteration)
-        int ln = (l + 1 < L) ? l + 1 : 0;
-        int kn = ls[ln] + threadIdx.x;
-        bool hn = kn < ls[ln + 1];
-        int rn = hn ? sched[kn] : 0;
-        RowRec<R> nxt;
-        if (hn) nxt = W.rows[rn];
-        if (have) {
-          R lam;
-          gs_row(W, cur, rcur, &lam, vl, wl, adl);
-          W.rows[rcur].lambda = lam;
+    double part = 0.0;
+    for (;;) {
+#pragma unroll
+      for (int s = 0; s < D; s++) {
+        const int k = qk[s];
+        const bool v = qv[s];
+        const R lnew = gs_row<R>(q[s], ql[s], v, v ? k : dummy, vw, lam, adl, part);
+#pragma unroll
+        for (int s2 = 0; s2 < D; s2++)
+          if (s2 != s && v && qv[s2] && qk[s2] == k) ql[s2] = lnew;
+        {  // the slot's next row (D steps on)
+          int kn = ls[pl] + tid;
+          bool vn = kn < ls[pl + 1];
+          int kc = vn ? kn : 0;
+          gs_load<R>(gsr, kc, q[s]);
+          R lm = lam[kc];
+          ql[s] = (v && vn && kc == k) ? lnew : lm;
+          qk[s] = kc;
+          qv[s] = vn;
+          pl = (pl + 1 == L) ? 0 : pl + 1;
         }
-        for (int k2 = ls[l] + threadIdx.x + blockDim.x; k2 < ls[l + 1]; k2 += blockDim.x) {
-          int r2 = sched[k2];
-          RowRec<R> o = W.rows[r2];
-          R lam;
-          gs_row(W, o, r2, &lam, vl, wl, adl);
-          W.rows[r2].lambda = lam;
+        // further rows of this thread in a level wider than the group
+        if (uni(ls[l + 1] - ls[l]) > bd) {
+          for (int k2 = ls[l] + tid + bd; k2 < ls[l + 1]; k2 += bd) {
+            GsRow<R> r;
+            gs_load<R>(gsr, k2, r);
+            (void)gs_row<R>(r, lam[k2], true, k2, vw, lam, adl, part);
+          }
         }
-        __syncthreads();
-        have = hn; rcur = rn;
-        if (hn) {
-          cur = nxt;
-          // lambda of the prefetched row may have been written after the prefetch was issued
-          // (it is the same row when the level holds a single row for this thread and L == 1)
-          if (L == 1) cur.lambda = W.rows[rn].lambda;
+        if constexpr (LDS) g.lds_sync();   // the next level reads what this one wrote
+        else g.sync();
+        if (++l == L) {   // end of a sweep: GSSolver's early-exit test
+          l = 0;
+          double Dsum = g.sum(part, shd);
+          int dec = gs_decide<R>(Dsum, E, tol2);
+          if (uni(dec) < 0) {
+            g.sync();   // every |dlambda| of the sweep is stored
+            int vv = 0;
+            if (tid == 0) {
+              R tot = 0;
+              for (int r = 0; r < E; r++) tot += adl[W.rpos[r]];
+              vv = tot * tot < tol2 ? 1 : 0;
+            }
+            dec = g.bcast0(vv, sh);
+          }
+          part = 0.0;
+          if (uni(dec) || ++it == iters) goto swept;
         }
       }
-      // fixed-order sum: each thread sums a contiguous block of rows, then a tree over threads
-      int per = (E + blockDim.x - 1) / blockDim.x;
-      int r0 = threadIdx.x * per, r1 = min(E, r0 + per);
-      R part = 0;
-      for (int r = r0; r < r1; r++) part += adl[r];
-      R tot = block_sum<R>(part, shr);
-      if (tot * tot < W.tol2) break;
     }
+  swept:
+    g.sync();
+    // Equation.multiplier and the capture read lambda in solver order
+    for (int k = tid; k < E; k += bd) W.rows[W.sched[k]].lambda = lam[k];
   }
-  if (vl != W.vl) {  // hand the lambda velocities to st_post
-    for (int i = threadIdx.x; i < 3 * nb; i += blockDim.x) { W.vl[i] = vl[i]; W.wl[i] = wl[i]; }
+  g.sync();
+  // hand the lambda velocities to st_post
+  for (int i = tid; i < nb; i += bd) {
+    float4 v = ((const float4*)vw)[2 * i], w = ((const float4*)vw)[2 * i + 1];
+    S3<R>(W.vl, i, mk3(v.x, v.y, v.z));
+    S3<R>(W.wl, i, mk3(w.x, w.y, w.z));
   }
-  if (threadIdx.x == 0) W.counters[C_ITERS] = it;
-  __syncthreads();
+  if (tid == 0) W.counters[C_ITERS] = it;
+  g.sync();
 }
 
 // ---- velocities, damping, integration ---------------------------------------------------------

@@ -14,6 +14,17 @@
 
 namespace chp {
 
+// Pointers into a world slab.  The device pass declares them global (address space 1) so that
+// every access compiles to global_* memory instructions rather than flat ones: flat accesses
+// also count as LDS operations, and a wait for an LDS result would then wait for every
+// outstanding global load as well.  The host pass sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+#else
+template <typename T> using gptr = T*;
+#endif
+#define CHP_G(T) gptr<T>
+
 enum : int {
   BF_RESPONSE = 1,       // Body.collisionResponse
   BF_TRIGGER = 2,        // Body.isTrigger
@@ -53,45 +64,47 @@ template <typename R> struct DevWorld {
   float pad1;
   R fric_grav_len, tol2;
   // bodies (FP32 vectors)
-  float *pos, *quat, *vel, *avel, *force, *torque, *iiw, *inv_inertia, *lin_factor, *ang_factor;
-  float *aabb_lo, *aabb_hi;
-  R *mass, *inv_mass, *brad, *ldamp, *adamp;
-  int *type, *flags, *group, *mask, *material, *shape_first, *shape_count;
-  int* order;  // SAPBroadphase.axisList (body indices), persistent across steps
+  CHP_G(float) pos, quat, vel, avel, force, torque, iiw, inv_inertia, lin_factor, ang_factor;
+  CHP_G(float) aabb_lo, aabb_hi;
+  CHP_G(R) mass, inv_mass, brad, ldamp, adamp;
+  CHP_G(int) type, flags, group, mask, material, shape_first, shape_count;
+  CHP_G(int) order;  // SAPBroadphase.axisList (body indices), persistent across steps
   // body-shape instances
-  int *si_type, *si_hull, *si_hf, *si_response, *si_group, *si_mask, *si_material;
-  float *si_off, *si_ori, *si_half;
-  R *si_radius, *si_bsr;
+  CHP_G(int) si_type, si_hull, si_hf, si_response, si_group, si_mask, si_material;
+  CHP_G(float) si_off, si_ori, si_half;
+  CHP_G(R) si_radius, si_bsr;
   // hull table
-  int *h_vstart, *h_vcount, *h_fstart, *h_fcount, *h_estart, *h_ecount, *h_axes;
-  R* h_bsr;
-  float *h_verts, *h_fnormal, *h_edges;
-  int *f_istart, *f_idx;
+  CHP_G(int) h_vstart, h_vcount, h_fstart, h_fcount, h_estart, h_ecount, h_axes;
+  CHP_G(R) h_bsr;
+  CHP_G(float) h_verts, h_fnormal, h_edges;
+  CHP_G(int) f_istart, f_idx;
   // heightfields
   int nhf, pad2;
-  int *hf_nx, *hf_ny, *hf_dstart, *hf_pillar_hull;
-  R *hf_min, *hf_max, *hf_es, *hf_data;
-  float* hf_pillar_off;  // per pillar (x, y, z)
+  CHP_G(int) hf_nx, hf_ny, hf_dstart, hf_pillar_hull;
+  CHP_G(R) hf_min, hf_max, hf_es, hf_data;
+  CHP_G(float) hf_pillar_off;  // per pillar (x, y, z)
   // materials
-  R *mat_friction, *mat_restitution;
-  int* cm_table;   // nmat*nmat -> contact material index, -1 none
-  R* cm_params;    // 6 per contact material (friction, restitution, cs, cr, fs, fr); [0] = default
+  CHP_G(R) mat_friction, mat_restitution;
+  CHP_G(int) cm_table;   // nmat*nmat -> contact material index, -1 none
+  CHP_G(R) cm_params;    // 6 per contact material (friction, restitution, cs, cr, fs, fr); [0] = default
   // constraints (6 equation slots each)
-  int *c_type, *c_a, *c_b, *c_collide, *c_row_off, *c_eq_mask;
-  float *c_vec;    // 12 vec3 per constraint: pivotA, pivotB, axisA, axisB, xA, xB, yA, yB, zA, zB
-  R *c_eq;         // 6 slots x (minF, maxF, cos, targetVel) + distance
+  CHP_G(int) c_type, c_a, c_b, c_collide, c_row_off, c_eq_mask;
+  CHP_G(float) c_vec;    // 12 vec3 per constraint: pivotA, pivotB, axisA, axisB, xA, xB, yA, yB, zA, zB
+  CHP_G(R) c_eq;         // 6 slots x (minF, maxF, cos, targetVel) + distance
   R c_spook[3];    // a, b, eps of the Equation constructor (1e7, 4, 1/60)
   // per-step scratch
-  float *feff, *vl, *wl, *iiws;
-  R* ims;
-  int *pair_i, *pair_j, *cnt, *tmp;
-  int *pc_cnt, *pc_base, *pf_cnt, *pc_off, *pf_off;
-  ContactRec<R>* contacts;
-  RowRec<R>* rows;
-  int *level, *sched, *lvl_start, *last_level, *row_ab, *bcnt, *blist;
-  R* absdl;
-  unsigned long long* sort_keys;
-  int* counters;   // [0] status [1] pairs [2] contacts [3] rows [4] iterations [5] levels [6] staging
+  CHP_G(float) feff, vl, wl, iiws;
+  CHP_G(R) ims;
+  CHP_G(int) pair_i, pair_j, cnt, tmp;
+  CHP_G(int) pc_cnt, pc_base, pf_cnt, pc_off, pf_off;
+  CHP_G(ContactRec<R>) contacts;
+  CHP_G(RowRec<R>) rows;
+  CHP_G(int) level, sched, lvl_start, last_level, row_ab, rpos;
+  CHP_G(R) absdl, glam;   // per row position in level order: |dlambda| of the last sweep, lambda
+  CHP_G(void) gsr;   // GS records in level order (24 R-sized words per row)
+  CHP_G(float) vw;   // vlambda / wlambda, 8 floats per body, when the world is too big for LDS
+  CHP_G(unsigned long long) sort_keys;
+  CHP_G(int) counters;   // [0] status [1] pairs [2] contacts [3] rows [4] iterations [5] levels [6] staging
 };
 
 }  // namespace chp
