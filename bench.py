@@ -75,6 +75,22 @@ def build_worlds(workload, nworlds, seed0=0):
     return recs, dyn, desc, dt
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime (its copy kernel reads and
+    writes nbytes): the calibration point for the HBM byte counters."""
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log("copy probe: %d bytes device to device" % nbytes)
+
+
 def measured_traffic(workload, precision, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     profiles/<kernel>_<workload>_traffic.md (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the
@@ -237,14 +253,7 @@ def main():
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib:
-        import torch
-        n = args.copy_probe_mib << 18
-        x = torch.ones(n, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
-        log("copy probe: %d MiB read, %d MiB written" % (args.copy_probe_mib, args.copy_probe_mib))
+        copy_probe(args.copy_probe_mib << 20)
 
 
 if __name__ == "__main__":

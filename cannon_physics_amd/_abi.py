@@ -222,6 +222,44 @@ class Row(C.Structure):
     ]
 
 
+class Equation(C.Structure):
+    """chp_equation: an equation as the Narrowphase slot hands it to the Solver slot"""
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("body_i", C.c_int32),
+        ("body_j", C.c_int32),
+        ("reserved", C.c_int32),
+        ("ri", F3),
+        ("rj", F3),
+        ("n", F3),
+        ("pad", C.c_float),
+        ("min_force", C.c_double),
+        ("max_force", C.c_double),
+        ("a", C.c_double),
+        ("b", C.c_double),
+        ("eps", C.c_double),
+        ("restitution", C.c_double),
+        ("angle_cos", C.c_double),
+        ("target_velocity", C.c_double),
+    ]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("error", C.c_int32),
+        ("steps_done", C.c_int32),
+        ("steps_total", C.c_int32),
+        ("num_bodies", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("wait_ms", C.c_double),
+        ("latency_ms", C.c_double),
+    ]
+
+
 class StepStats(C.Structure):
     _fields_ = [
         ("pairs", C.c_uint32),
@@ -252,4 +290,6 @@ STRUCT_SIZES = {
     "chp_body_info": C.sizeof(BodyInfo),
     "chp_row": C.sizeof(Row),
     "chp_step_stats": C.sizeof(StepStats),
+    "chp_equation": C.sizeof(Equation),
+    "chp_request_status": C.sizeof(RequestStatus),
 }

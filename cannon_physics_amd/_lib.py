@@ -54,6 +54,14 @@ def load():
         "chp_broadphase": (i32, [ctx_p, u32, P(i32), P(i32), u32, P(u32)]),
         "chp_step_capture": (i32, [ctx_p, u32, dbl, P(i32), P(i32), u32, P(u32), P(A.Row), u32,
                                    P(u32), P(i32)]),
+        "chp_narrowphase": (i32, [ctx_p, u32, P(i32), P(i32), u32, dbl, P(A.Equation), u32, P(u32), P(u32),
+                                  P(u32)]),
+        "chp_solve": (i32, [ctx_p, u32, P(A.Equation), u32, dbl, P(i32), P(dbl)]),
+        "chp_serve_submit": (i32, [ctx_p, P(A.Scene), dbl, i32, P(u64)]),
+        "chp_serve_tick": (i32, [ctx_p, i32, P(u32), P(u32)]),
+        "chp_serve_status": (i32, [ctx_p, u64, P(A.RequestStatus)]),
+        "chp_serve_result": (i32, [ctx_p, u64, P(A.BodyState), u32]),
+        "chp_serve_cancel": (i32, [ctx_p, u64]),
         "chp_quat_from_euler": (None, [dbl, dbl, dbl, P(C.c_float)]),
         "chp_abi_struct_size": (u32, [C.c_char_p]),
         "chp_scene_inspect": (i32, [P(A.Scene), C.c_int, P(A.BodyInfo), P(A.BodyState), u32, C.c_char_p, u32]),

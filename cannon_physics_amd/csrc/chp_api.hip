@@ -4,6 +4,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <chrono>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -160,6 +163,27 @@ __global__ __launch_bounds__(CHP_BLOCK) void k_broadphase(const DevWorld<R>* __r
   }
 }
 
+// Narrowphase slot: contacts for the pairs in the world's pair slab, then the equations
+template <typename R>
+__global__ __launch_bounds__(64) void k_np_slot(const DevWorld<R>* __restrict__ worlds, int id, double dt,
+                                                chp_equation* __restrict__ out, int cap) {
+  __shared__ int sh[64];
+  const DevWorld<R>& W = worlds[id];
+  st_narrow(W, (R)dt);
+  st_equations(W, (R)dt, out, cap, sh);
+}
+
+// Solver slot: rows from given equations; then k_levels / k_gs; then the velocity update
+template <typename R>
+__global__ __launch_bounds__(256) void k_solve_rows(const DevWorld<R>* __restrict__ worlds, int id,
+                                                    const chp_equation* __restrict__ eq, int n, double dt) {
+  st_solve_rows(worlds[id], eq, n, (R)dt);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_solve_finish(const DevWorld<R>* __restrict__ worlds, int id) {
+  st_solve_finish(worlds[id]);
+}
+
 __global__ void k_gather_status(const int* const* counters, int n, int* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -291,6 +315,23 @@ struct chp_ctx {
   std::vector<hipEvent_t> sev;               // 7 events per step
   std::vector<double> stage_ms;              // accumulated per stage over the last chp_step
   int solve_block = 256;   // workgroup of k_levels / k_gs: one large world or blockDim/64 small ones
+  chp_equation* d_eq = nullptr;  // staging of the narrowphase / solver slots
+  size_t d_eq_cap = 0;
+  // serving
+  struct Request {
+    int state = CHP_REQ_QUEUED, error = CHP_OK;
+    int steps_done = 0, steps_total = 0;
+    double dt = 0;
+    uint32_t nb = 0, world = 0;
+    std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
+    WorldSlot proto;               // host part of the world, installed at admission
+    std::vector<chp_body_state> result;
+    std::string msg;
+  };
+  std::map<uint64_t, Request> reqs;
+  std::deque<uint64_t> queue;      // waiting for admission, in arrival order
+  std::vector<uint64_t> running;
+  uint64_t next_ticket = 1;
   int lds_cap_bodies = 2048;
 };
 
@@ -420,15 +461,17 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   return CHP_OK;
 }
 
-template <typename R> static int world_create_t(chp_ctx* ctx, const chp_scene* scene, uint32_t* id) {
-  HostWorld<R> H;
+// constructor replay on the host: the world's static tables and initial state, not yet on the
+// device
+template <typename R> static int prepare_world(chp_ctx* ctx, const chp_scene* scene, WorldSlot& S) {
+  auto Hp = std::make_shared<HostWorld<R>>();
+  HostWorld<R>& H = *Hp;
   try {
     Builder<R> B(H, *scene);
     B.run();
   } catch (const std::exception& ex) {
     return fail(ctx, CHP_E_INVALID, ex.what());
   }
-  WorldSlot S;
   S.live = true;
   for (uint32_t i = 0; i < scene->num_bodies; i++) {
     S.lin_damp.push_back(scene->bodies[i].linear_damping);
@@ -440,12 +483,19 @@ template <typename R> static int world_create_t(chp_ctx* ctx, const chp_scene* s
     bi.type = H.type[i];
     S.info.push_back(bi);
   }
+  S.nb = H.nb;
+  S.host = Hp;
+  return CHP_OK;
+}
+
+// slabs on the device and a world id
+template <typename R> static int install_world(chp_ctx* ctx, WorldSlot&& S, uint32_t* id) {
+  const HostWorld<R>& H = *(const HostWorld<R>*)S.host.get();
   int pc, cc, rc;
   default_caps(H.nb, (int)H.hf_nx.size(), pc, cc, rc);
   rc = std::max(rc, H.ncrows + 3 * cc);
   int r = build_slot<R>(ctx, S, H, pc, cc, rc);
   if (r != CHP_OK) return r;
-  S.host = std::make_shared<HostWorld<R>>(std::move(H));
   size_t slot = ctx->worlds.size();
   for (size_t k = 0; k < ctx->worlds.size(); k++)
     if (!ctx->worlds[k].live) { slot = k; break; }
@@ -453,6 +503,21 @@ template <typename R> static int world_create_t(chp_ctx* ctx, const chp_scene* s
   ctx->worlds[slot] = std::move(S);
   *id = (uint32_t)slot;
   return CHP_OK;
+}
+
+template <typename R> static int world_create_t(chp_ctx* ctx, const chp_scene* scene, uint32_t* id) {
+  WorldSlot S;
+  int r = prepare_world<R>(ctx, scene, S);
+  if (r != CHP_OK) return r;
+  return install_world<R>(ctx, std::move(S), id);
+}
+
+static void release_world(chp_ctx* ctx, uint32_t id) {
+  WorldSlot& S = ctx->worlds[id];
+  ctx->arena.release(S.mem, S.bytes);
+  ctx->in_use -= S.bytes;
+  S = WorldSlot();
+  ctx->worlds_dirty = true;
 }
 
 static bool valid_world(chp_ctx* ctx, uint32_t id) { return ctx && id < ctx->worlds.size() && ctx->worlds[id].live; }
@@ -562,7 +627,10 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
   return CHP_OK;
 }
 
-template <typename R> static int step_t(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt, int nsteps, bool wait) {
+// launches nsteps steps of the listed worlds (largest first); with wait, synchronises and, when
+// status is given, returns every listed world's status word in list order
+template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt, int nsteps,
+                                           bool wait, std::vector<int>* status) {
   int r = prepare<R>(ctx, ids, n, dt);
   if (r != CHP_OK) return r;
   if (ctx->d_list_cap < n) {
@@ -599,6 +667,19 @@ template <typename R> static int step_t(chp_ctx* ctx, const uint32_t* ids, uint3
         HIPCHK(ctx, hipEventElapsedTime(&f, ctx->sev[s * (K_NSTAGE + 1) + k], ctx->sev[s * (K_NSTAGE + 1) + k + 1]));
         ctx->stage_ms[k] += f;
       }
+  if (status) {
+    std::vector<int> st;
+    r = read_status(ctx, ids, n, st);
+    if (r != CHP_OK) return r;
+    status->resize(n);
+    for (uint32_t k = 0; k < n; k++) (*status)[k] = st[k * C_COUNT + C_STATUS];
+  }
+  return CHP_OK;
+}
+
+template <typename R> static int step_t(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt, int nsteps, bool wait) {
+  int r = run_steps<R>(ctx, ids, n, dt, nsteps, wait, nullptr);
+  if (r != CHP_OK || (!wait && !ctx->profile)) return r;
   return check_status(ctx, ids, n);
 }
 
@@ -759,6 +840,207 @@ static int capture_t(chp_ctx* ctx, uint32_t id, double dt, int32_t* pi, int32_t*
 }
 
 
+static int ensure_eq(chp_ctx* ctx, size_t n) {
+  if (ctx->d_eq_cap >= n) return CHP_OK;
+  if (ctx->d_eq) (void)hipFree(ctx->d_eq);
+  ctx->d_eq = nullptr;
+  ctx->d_eq_cap = 0;
+  HIPCHK(ctx, hipMalloc(&ctx->d_eq, std::max<size_t>(n, 1) * sizeof(chp_equation)));
+  ctx->d_eq_cap = std::max<size_t>(n, 1);
+  return CHP_OK;
+}
+
+template <typename R>
+static int narrowphase_t(chp_ctx* ctx, uint32_t id, const int32_t* pi, const int32_t* pj, uint32_t np, double dt,
+                         chp_equation* out, uint32_t cap, uint32_t* neq, uint32_t* nf, uint32_t* nc) {
+  WorldSlot& S = ctx->worlds[id];
+  if (np > (uint32_t)S.pair_cap)
+    return fail(ctx, CHP_E_CAPACITY, "more pairs than the world's pair slab (" + std::to_string(S.pair_cap) +
+                                         "); raise it with chp_world_set_capacity");
+  for (uint32_t k = 0; k < np; k++)
+    if (pi[k] < 0 || pj[k] < 0 || pi[k] >= S.nb || pj[k] >= S.nb || pi[k] == pj[k])
+      return fail(ctx, CHP_E_INVALID, "pair " + std::to_string(k) + " names an invalid body");
+  int r = prepare<R>(ctx, &id, 1, dt);
+  if (r != CHP_OK) return r;
+  r = ensure_eq(ctx, S.row_cap);
+  if (r != CHP_OK) return r;
+  DevWorld<R>& D = dev_of<R>(S);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (np) {
+    HIPCHK(ctx, hipMemcpy(D.pair_i, pi, np * 4, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(D.pair_j, pj, np * 4, hipMemcpyHostToDevice));
+  }
+  int c[C_COUNT];
+  HIPCHK(ctx, hipMemcpy(c, D.counters, sizeof c, hipMemcpyDeviceToHost));
+  c[C_PAIRS] = (int)np; c[C_STATUS] = 0; c[C_STAGING] = 0; c[C_ROWS] = 0;
+  HIPCHK(ctx, hipMemcpy(D.counters, c, sizeof c, hipMemcpyHostToDevice));
+  k_np_slot<R><<<1, 64, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, (int)id, dt, ctx->d_eq, S.row_cap);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(c, D.counters, sizeof c, hipMemcpyDeviceToHost));
+  if (c[C_STATUS]) {
+    int st = c[C_STATUS];
+    c[C_STATUS] = 0;
+    HIPCHK(ctx, hipMemcpy(D.counters, c, sizeof c, hipMemcpyHostToDevice));
+    return fail(ctx, CHP_E_CAPACITY, "contact or row slab too small (status " + std::to_string(st) +
+                                         "); raise it with chp_world_set_capacity");
+  }
+  *neq = (uint32_t)c[C_ROWS];
+  *nf = (uint32_t)c[C_FROWS];
+  *nc = (uint32_t)c[C_CROWS];
+  if (*neq > cap) return fail(ctx, CHP_E_CAPACITY, "equation buffer too small: need " + std::to_string(*neq));
+  if (*neq) HIPCHK(ctx, hipMemcpy(out, ctx->d_eq, *neq * sizeof(chp_equation), hipMemcpyDeviceToHost));
+  return CHP_OK;
+}
+
+template <typename R>
+static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t n, double dt, int32_t* iters,
+                   double* mult) {
+  WorldSlot& S = ctx->worlds[id];
+  if (n > (uint32_t)S.row_cap)
+    return fail(ctx, CHP_E_CAPACITY, "more equations than the world's row slab (" + std::to_string(S.row_cap) +
+                                         "); raise it with chp_world_set_capacity");
+  for (uint32_t k = 0; k < n; k++) {
+    const chp_equation& e = eqs[k];
+    if (e.body_i < 0 || e.body_j < 0 || e.body_i >= S.nb || e.body_j >= S.nb || e.kind < CHP_ROW_FRICTION ||
+        e.kind > CHP_ROW_MOTOR)
+      return fail(ctx, CHP_E_INVALID, "equation " + std::to_string(k) + " is malformed");
+  }
+  int r = prepare<R>(ctx, &id, 1, dt);
+  if (r != CHP_OK) return r;
+  r = ensure_eq(ctx, n);
+  if (r != CHP_OK) return r;
+  if (ctx->d_list_cap < 1) {
+    HIPCHK(ctx, hipMalloc(&ctx->d_list, sizeof(int)));
+    ctx->d_list_cap = 1;
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (n) HIPCHK(ctx, hipMemcpy(ctx->d_eq, eqs, n * sizeof(chp_equation), hipMemcpyHostToDevice));
+  int one = (int)id;
+  HIPCHK(ctx, hipMemcpy(ctx->d_list, &one, sizeof(int), hipMemcpyHostToDevice));
+  const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
+  const int bd = ctx->solve_block, nbig = S.nb > SMALL_NB ? 1 : 0;
+  hipStream_t st = ctx->stream;
+  k_solve_rows<R><<<1, 256, 0, st>>>(dw, (int)id, ctx->d_eq, (int)n, dt);
+  k_levels<R><<<1, bd, lds_levels(bd), st>>>(dw, ctx->d_list, nbig, 1);
+  k_gs<R><<<1, bd, lds_gs(bd), st>>>(dw, ctx->d_list, nbig, 1);
+  k_solve_finish<R><<<1, 256, 0, st>>>(dw, (int)id);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  DevWorld<R>& D = dev_of<R>(S);
+  int c[C_COUNT];
+  HIPCHK(ctx, hipMemcpy(c, D.counters, sizeof c, hipMemcpyDeviceToHost));
+  *iters = c[C_ITERS];
+  if (mult && n) {
+    std::vector<RowRec<R>> rr(n);
+    HIPCHK(ctx, hipMemcpy(rr.data(), D.rows, n * sizeof(RowRec<R>), hipMemcpyDeviceToHost));
+    R invDt = (R)1 / (R)dt;   // gs_solver.dart:124-129
+    for (uint32_t k = 0; k < n; k++) mult[k] = (double)(rr[k].lambda * invDt);
+  }
+  return CHP_OK;
+}
+
+// ---- serving ----
+static double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+template <typename R> static int serve_submit_t(chp_ctx* ctx, const chp_scene* scene, double dt, int nsteps, uint64_t* t) {
+  chp_ctx::Request q;
+  int r = prepare_world<R>(ctx, scene, q.proto);
+  if (r != CHP_OK) return r;
+  q.dt = dt;
+  q.steps_total = nsteps;
+  q.nb = scene->num_bodies;
+  q.t_submit = std::chrono::steady_clock::now();
+  uint64_t ticket = ctx->next_ticket++;
+  ctx->reqs.emplace(ticket, std::move(q));
+  ctx->queue.push_back(ticket);
+  *t = ticket;
+  return CHP_OK;
+}
+
+template <typename R> static void serve_retire(chp_ctx* ctx, uint64_t ticket, int state, int error, const std::string& msg) {
+  chp_ctx::Request& q = ctx->reqs[ticket];
+  if (state == CHP_REQ_DONE) {
+    q.result.resize(ctx->worlds[q.world].nb);
+    if (state_io<R>(ctx, q.world, q.result.data(), (uint32_t)q.result.size(), true) != CHP_OK) {
+      state = CHP_REQ_FAILED;
+      error = CHP_E_HIP;
+    }
+  }
+  release_world(ctx, q.world);
+  q.state = state;
+  q.error = error;
+  q.msg = msg;
+  q.t_done = std::chrono::steady_clock::now();
+}
+
+template <typename R> static int serve_tick_t(chp_ctx* ctx, int max_steps, uint32_t* nrun, uint32_t* nq) {
+  // admission in arrival order while the HBM budget allows
+  while (!ctx->queue.empty()) {
+    uint64_t ticket = ctx->queue.front();
+    chp_ctx::Request& q = ctx->reqs[ticket];
+    uint32_t id;
+    int r = install_world<R>(ctx, std::move(q.proto), &id);
+    if (r == CHP_E_OOM && !ctx->running.empty()) break;   // wait for memory to free up
+    ctx->queue.pop_front();
+    if (r != CHP_OK) {
+      q.state = CHP_REQ_FAILED;
+      q.error = r;
+      q.msg = ctx->err;
+      q.t_done = std::chrono::steady_clock::now();
+      continue;
+    }
+    q.world = id;
+    q.state = CHP_REQ_RUNNING;
+    q.t_admit = std::chrono::steady_clock::now();
+    q.proto = WorldSlot();
+    ctx->running.push_back(ticket);
+    if (q.steps_total <= 0) {
+      serve_retire<R>(ctx, ticket, CHP_REQ_DONE, CHP_OK, "");
+      ctx->running.pop_back();
+    }
+  }
+  // advance: the running requests of each dt together, by the steps the nearest finisher needs
+  int left = max_steps;
+  while (left > 0 && !ctx->running.empty()) {
+    double dt = ctx->reqs[ctx->running[0]].dt;
+    std::vector<uint64_t> group;
+    int k = left;
+    for (uint64_t t : ctx->running) {
+      const chp_ctx::Request& q = ctx->reqs[t];
+      if (q.dt != dt) continue;
+      group.push_back(t);
+      k = std::min(k, q.steps_total - q.steps_done);
+    }
+    std::vector<uint32_t> ids;
+    for (uint64_t t : group) ids.push_back(ctx->reqs[t].world);
+    std::vector<int> status;
+    int r = run_steps<R>(ctx, ids.data(), (uint32_t)ids.size(), dt, k, true, &status);
+    if (r != CHP_OK) return r;
+    for (size_t g = 0; g < group.size(); g++) {
+      chp_ctx::Request& q = ctx->reqs[group[g]];
+      if (status[g]) {
+        int err = (status[g] & WS_NONFINITE) ? CHP_E_NONFINITE : CHP_E_CAPACITY;
+        serve_retire<R>(ctx, group[g], CHP_REQ_FAILED, err,
+                        err == CHP_E_NONFINITE ? "non-finite state" : "a per-world slab overflowed");
+      } else {
+        q.steps_done += k;
+        if (q.steps_done >= q.steps_total) serve_retire<R>(ctx, group[g], CHP_REQ_DONE, CHP_OK, "");
+      }
+    }
+    std::vector<uint64_t> still;
+    for (uint64_t t : ctx->running)
+      if (ctx->reqs[t].state == CHP_REQ_RUNNING) still.push_back(t);
+    ctx->running.swap(still);
+    left -= k;
+  }
+  if (nrun) *nrun = (uint32_t)ctx->running.size();
+  if (nq) *nq = (uint32_t)ctx->queue.size();
+  return CHP_OK;
+}
+
 template <typename R> static int inspect_t(const chp_scene* scene, chp_body_info* info, chp_body_state* state,
                                            uint32_t cap, char* err, uint32_t errcap) {
   HostWorld<R> H;
@@ -827,6 +1109,7 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   if (ctx->d_list) (void)hipFree(ctx->d_list);
   if (ctx->d_status) (void)hipFree(ctx->d_status);
   if (ctx->d_cptrs) (void)hipFree(ctx->d_cptrs);
+  if (ctx->d_eq) (void)hipFree(ctx->d_eq);
   for (hipEvent_t e : ctx->sev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
@@ -975,6 +1258,99 @@ int chp_step_capture(chp_ctx* ctx, uint32_t id, double dt, int32_t* pi, int32_t*
              : capture_t<float>(ctx, id, dt, pi, pj, pair_cap, npairs, rows, row_cap, nrows, iterations);
 }
 
+int chp_narrowphase(chp_ctx* ctx, uint32_t id, const int32_t* pi, const int32_t* pj, uint32_t npairs, double dt,
+                    chp_equation* out, uint32_t cap, uint32_t* neq, uint32_t* nf, uint32_t* nc) {
+  if (!valid_world(ctx, id) || (npairs && (!pi || !pj)) || !neq || !nf || !nc || (cap && !out))
+    return fail(ctx, CHP_E_INVALID, "bad argument");
+  if (!(dt > 0)) return fail(ctx, CHP_E_INVALID, "dt must be > 0");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? narrowphase_t<double>(ctx, id, pi, pj, npairs, dt, out, cap, neq, nf, nc)
+                                             : narrowphase_t<float>(ctx, id, pi, pj, npairs, dt, out, cap, neq, nf, nc);
+}
+
+int chp_solve(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t n, double dt, int32_t* iterations,
+              double* multipliers) {
+  if (!valid_world(ctx, id) || (n && !eqs) || !iterations) return fail(ctx, CHP_E_INVALID, "bad argument");
+  if (!(dt > 0)) return fail(ctx, CHP_E_INVALID, "dt must be > 0");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? solve_t<double>(ctx, id, eqs, n, dt, iterations, multipliers)
+                                             : solve_t<float>(ctx, id, eqs, n, dt, iterations, multipliers);
+}
+
+int chp_serve_submit(chp_ctx* ctx, const chp_scene* scene, double dt, int32_t nsteps, uint64_t* ticket) {
+  if (!ctx || !scene || !ticket) return fail(ctx, CHP_E_INVALID, "null argument");
+  if (!(dt > 0) || nsteps < 0) return fail(ctx, CHP_E_INVALID, "dt must be > 0 and nsteps >= 0");
+  if (scene->num_bodies == 0) return fail(ctx, CHP_E_INVALID, "a world needs at least one body");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  return ctx->precision == CHP_PRECISION_F64 ? serve_submit_t<double>(ctx, scene, dt, nsteps, ticket)
+                                             : serve_submit_t<float>(ctx, scene, dt, nsteps, ticket);
+}
+
+int chp_serve_tick(chp_ctx* ctx, int32_t max_steps, uint32_t* nrunning, uint32_t* nqueued) {
+  if (!ctx || max_steps < 0) return fail(ctx, CHP_E_INVALID, "bad argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? serve_tick_t<double>(ctx, max_steps, nrunning, nqueued)
+                                             : serve_tick_t<float>(ctx, max_steps, nrunning, nqueued);
+}
+
+int chp_serve_status(chp_ctx* ctx, uint64_t ticket, chp_request_status* out) {
+  if (!ctx || !out) return fail(ctx, CHP_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->reqs.find(ticket);
+  if (it == ctx->reqs.end()) return fail(ctx, CHP_E_INVALID, "unknown ticket");
+  const chp_ctx::Request& q = it->second;
+  memset(out, 0, sizeof *out);
+  out->state = q.state;
+  out->error = q.error;
+  out->steps_done = q.steps_done;
+  out->steps_total = q.steps_total;
+  out->num_bodies = q.nb;
+  auto now = std::chrono::steady_clock::now();
+  out->wait_ms = ms_between(q.t_submit, q.state == CHP_REQ_QUEUED ? now : q.t_admit);
+  if (q.state == CHP_REQ_DONE || q.state == CHP_REQ_FAILED) out->latency_ms = ms_between(q.t_submit, q.t_done);
+  return CHP_OK;
+}
+
+int chp_serve_result(chp_ctx* ctx, uint64_t ticket, chp_body_state* out, uint32_t cap) {
+  if (!ctx || !out) return fail(ctx, CHP_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->reqs.find(ticket);
+  if (it == ctx->reqs.end()) return fail(ctx, CHP_E_INVALID, "unknown ticket");
+  chp_ctx::Request& q = it->second;
+  if (q.state == CHP_REQ_FAILED) {
+    int e = q.error;
+    std::string m = q.msg;
+    ctx->reqs.erase(it);
+    return fail(ctx, e, "request failed: " + m);
+  }
+  if (q.state != CHP_REQ_DONE) return fail(ctx, CHP_E_INVALID, "request not finished");
+  if (cap < q.result.size()) return fail(ctx, CHP_E_CAPACITY, "state buffer smaller than the body count");
+  memcpy(out, q.result.data(), q.result.size() * sizeof(chp_body_state));
+  ctx->reqs.erase(it);
+  return CHP_OK;
+}
+
+int chp_serve_cancel(chp_ctx* ctx, uint64_t ticket) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  auto it = ctx->reqs.find(ticket);
+  if (it == ctx->reqs.end()) return fail(ctx, CHP_E_INVALID, "unknown ticket");
+  chp_ctx::Request& q = it->second;
+  if (q.state == CHP_REQ_QUEUED) {
+    ctx->queue.erase(std::remove(ctx->queue.begin(), ctx->queue.end(), ticket), ctx->queue.end());
+  } else if (q.state == CHP_REQ_RUNNING) {
+    (void)hipStreamSynchronize(ctx->stream);
+    release_world(ctx, q.world);
+    ctx->running.erase(std::remove(ctx->running.begin(), ctx->running.end(), ticket), ctx->running.end());
+  }
+  ctx->reqs.erase(it);
+  return CHP_OK;
+}
+
 void chp_quat_from_euler(double x, double y, double z, float out[4]) {
   // Quat.setFromEuler, order xyz (quaternion.dart:191-232)
   double c1 = std::cos(x / 2), c2 = std::cos(y / 2), c3 = std::cos(z / 2);
@@ -1040,7 +1416,7 @@ uint32_t chp_abi_struct_size(const char* name) {
 #define SZ(t) if (n == #t) return (uint32_t)sizeof(t)
   SZ(chp_world_desc); SZ(chp_material_desc); SZ(chp_contact_material_desc); SZ(chp_shape_desc); SZ(chp_hull_desc);
   SZ(chp_heightfield_desc); SZ(chp_body_shape); SZ(chp_body_desc); SZ(chp_constraint_desc); SZ(chp_scene);
-  SZ(chp_body_state); SZ(chp_body_info); SZ(chp_row); SZ(chp_step_stats);
+  SZ(chp_body_state); SZ(chp_body_info); SZ(chp_row); SZ(chp_step_stats); SZ(chp_equation); SZ(chp_request_status);
 #undef SZ
   return 0;
 }

@@ -354,34 +354,34 @@ template <typename R> __device__ void contact_row(const DevWorld<R>& W, int r, i
 }
 // FrictionEquation.computeB (friction_equation.dart:19-48)
 template <typename R> __device__ void friction_row(const DevWorld<R>& W, int r, int a, int b, f3 ri, f3 rj, f3 t, R sb_,
-                                                   R se, R h, R slip) {
+                                                   R se, R h, R minF, R maxF) {
   f3 rixt = cross3<R>(ri, t), rjxt = cross3<R>(rj, t);
   f3 gsa = neg3(t), gra = neg3(rixt), gsb = t, grb = rjxt;
   R gw = gwv(W, a, b, gsa, gra, gsb, grb);
   R gm = giMf(W, a, b, gsa, gra, gsb, grb);
   R B = -gw * sb_ - h * gm;
-  put_row(W, r, CHP_ROW_FRICTION, a, b, gsa, gra, gsb, grb, B, se, -slip, slip);
+  put_row(W, r, CHP_ROW_FRICTION, a, b, gsa, gra, gsb, grb, B, se, minF, maxF);
 }
 // RotationalEquation / ConeEquation.computeB (rotational_equation.dart:34-58, cone_equation.dart:34-57)
 template <typename R> __device__ void rot_row(const DevWorld<R>& W, int r, int kind, int a, int b, f3 ni, f3 nj, R cosA,
-                                              R h, R minF, R maxF) {
+                                              R h, R minF, R maxF, R sa_, R sb_, R se) {
   f3 z = mk3(0.f, 0.f, 0.f);
   f3 gra = cross3<R>(nj, ni), grb = cross3<R>(ni, nj);
   R g = cosA - dot3<R>(ni, nj);
   R gw = gwv(W, a, b, z, gra, z, grb);
   R gm = giMf(W, a, b, z, gra, z, grb);
-  R B = -g * W.c_spook[0] - gw * W.c_spook[1] - h * gm;
-  put_row(W, r, kind, a, b, z, gra, z, grb, B, W.c_spook[2], minF, maxF);
+  R B = -g * sa_ - gw * sb_ - h * gm;
+  put_row(W, r, kind, a, b, z, gra, z, grb, B, se, minF, maxF);
 }
 // RotationalMotorEquation.computeB (rotational_motor_equation.dart:17-33)
 template <typename R> __device__ void motor_row(const DevWorld<R>& W, int r, int a, int b, f3 axA, f3 axB, R target, R h,
-                                                R minF, R maxF) {
+                                                R minF, R maxF, R sb_, R se) {
   f3 z = mk3(0.f, 0.f, 0.f);
   f3 gra = axA, grb = neg3(axB);
   R gw = gwv(W, a, b, z, gra, z, grb) - target;
   R gm = giMf(W, a, b, z, gra, z, grb);
-  R B = -gw * W.c_spook[1] - h * gm;
-  put_row(W, r, CHP_ROW_MOTOR, a, b, z, gra, z, grb, B, W.c_spook[2], minF, maxF);
+  R B = -gw * sb_ - h * gm;
+  put_row(W, r, CHP_ROW_MOTOR, a, b, z, gra, z, grb, B, se, minF, maxF);
 }
 
 // Constraint.update + the rows each joint feeds the solver (lib/constraints/)
@@ -408,20 +408,20 @@ template <typename R> __device__ void constraint_rows(const DevWorld<R>& W, int 
     f3 wA = qrot<R>(qa, v[2]), wB = qrot<R>(qb, v[3]);
     f3 t1, t2;
     tangents3<R>(wA, t1, t2);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, t1, wB, eq[12 + 2], h, eq[12], eq[13]);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, t2, wB, eq[16 + 2], h, eq[16], eq[17]);
-    if (mask & 32) motor_row(W, r++, a, b, qrot<R>(qa, v[2]), qrot<R>(qb, v[3]), eq[20 + 3], h, eq[20], eq[21]);
+    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, t1, wB, eq[12 + 2], h, eq[12], eq[13], sa_, sb_, se);
+    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, t2, wB, eq[16 + 2], h, eq[16], eq[17], sa_, sb_, se);
+    if (mask & 32) motor_row(W, r++, a, b, qrot<R>(qa, v[2]), qrot<R>(qb, v[3]), eq[20 + 3], h, eq[20], eq[21], sb_, se);
   } else if (type == CHP_CONSTRAINT_LOCK) {  // lock_constraint.dart:70-88
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[4]), qrot<R>(qb, v[7]), eq[14], h, eq[12], eq[13]);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[6]), qrot<R>(qb, v[9]), eq[18], h, eq[16], eq[17]);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[8]), qrot<R>(qb, v[5]), eq[22], h, eq[20], eq[21]);
+    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[4]), qrot<R>(qb, v[7]), eq[14], h, eq[12], eq[13], sa_, sb_, se);
+    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[6]), qrot<R>(qb, v[9]), eq[18], h, eq[16], eq[17], sa_, sb_, se);
+    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[8]), qrot<R>(qb, v[5]), eq[22], h, eq[20], eq[21], sa_, sb_, se);
   } else if (type == CHP_CONSTRAINT_CONE_TWIST) {  // cone_twist_constraint.dart:77-96
     f3 cA = qrot<R>(qa, v[2]), cB = qrot<R>(qb, v[3]);
-    rot_row(W, r++, CHP_ROW_CONE, a, b, cA, cB, eq[14], h, eq[12], eq[13]);
+    rot_row(W, r++, CHP_ROW_CONE, a, b, cA, cB, eq[14], h, eq[12], eq[13], sa_, sb_, se);
     f3 t1, t2, u1, u2;
     tangents3<R>(v[2], t1, t2);   // axisA.tangents(twist.axisA, twist.axisA): the second write wins
     tangents3<R>(v[3], u1, u2);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, t2), qrot<R>(qb, u2), eq[18], h, eq[16], eq[17]);
+    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, t2), qrot<R>(qb, u2), eq[18], h, eq[16], eq[17], sa_, sb_, se);
   }
 }
 
@@ -447,8 +447,8 @@ template <typename R> __device__ bool st_rows(const DevWorld<R>& W, R h, int* sh
         spook<R>(cm[4], cm[5], h, fa, fb, fe);
         f3 t1, t2;
         tangents3<R>(c.ni, t1, t2);
-        friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t1, fb, fe, h, c.slip);
-        friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t2, fb, fe, h, c.slip);
+        friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t1, fb, fe, h, -c.slip, c.slip);
+        friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t2, fb, fe, h, -c.slip, c.slip);
       }
       R ca, cb, ce;
       spook<R>(cm[2], cm[3], h, ca, cb, ce);
@@ -467,6 +467,104 @@ template <typename R> __device__ bool st_rows(const DevWorld<R>& W, R h, int* sh
   }
   __syncthreads();
   return true;
+}
+
+// ---- slot stages ---------------------------------------------------------------------------------
+// The equations World.internalStep hands the solver for the world's current pairs (the
+// Narrowphase slot with World's row assembly, world_class.dart:539-635, without the joints):
+// friction equations then contact equations in generation order, disabled and trigger ones
+// dropped, the body-material restitution override applied.  B and 1/C are the solver's.
+template <typename R> __device__ void st_equations(const DevWorld<R>& W, R h, chp_equation* out, int cap, int* sh) {
+  int P = W.counters[C_PAIRS];
+  int F = array_exscan(W.pf_cnt, P, sh);
+  int C = array_exscan(W.pc_off, P, sh);
+  if (F + C > cap) {
+    if (threadIdx.x == 0) { W.counters[C_ROWS] = F + C; ws_fail(W.counters, WS_ROWS); }
+    return;
+  }
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    int fo = W.pf_cnt[p], co = F + W.pc_off[p];
+    for (int k = 0; k < W.pc_cnt[p]; k++) {
+      const ContactRec<R>& c = W.contacts[W.pc_base[p] + k];
+      if (!(c.flags & 1)) continue;
+      const R* cm = W.cm_params + 6 * c.cm;
+      chp_equation e;
+      memset(&e, 0, sizeof e);
+      e.body_i = c.bi; e.body_j = c.bj;
+      e.ri[0] = c.ri.x; e.ri[1] = c.ri.y; e.ri[2] = c.ri.z;
+      e.rj[0] = c.rj.x; e.rj[1] = c.rj.y; e.rj[2] = c.rj.z;
+      if (c.flags & 2) {
+        R fa, fb, fe;
+        spook<R>(cm[4], cm[5], h, fa, fb, fe);
+        f3 t[2];
+        tangents3<R>(c.ni, t[0], t[1]);
+        e.kind = CHP_ROW_FRICTION;
+        e.min_force = (double)-c.slip; e.max_force = (double)c.slip;
+        e.a = (double)fa; e.b = (double)fb; e.eps = (double)fe;
+        for (int q = 0; q < 2; q++) {
+          e.n[0] = t[q].x; e.n[1] = t[q].y; e.n[2] = t[q].z;
+          out[fo++] = e;
+        }
+      }
+      R ca, cb, ce;
+      spook<R>(cm[2], cm[3], h, ca, cb, ce);
+      e.kind = CHP_ROW_CONTACT;
+      e.n[0] = c.ni.x; e.n[1] = c.ni.y; e.n[2] = c.ni.z;
+      e.min_force = 0.0; e.max_force = 1e6;
+      e.a = (double)ca; e.b = (double)cb; e.eps = (double)ce;
+      e.restitution = (double)c.restitution;
+      out[co++] = e;
+    }
+  }
+  if (threadIdx.x == 0) { W.counters[C_ROWS] = F + C; W.counters[C_FROWS] = F; W.counters[C_CROWS] = C; }
+  __syncthreads();
+}
+
+// GSSolver.solve's set-up for given equations (gs_solver.dart:27-75): solve masses, the applied
+// force as the body holds it (the World added gravity before calling the solver), and each
+// equation's B and 1/C from its own spook parameters.
+template <typename R> __device__ void st_solve_rows(const DevWorld<R>& W, const chp_equation* eq, int n, R h) {
+  for (int i = threadIdx.x; i < W.nb; i += blockDim.x) {
+    S3<R>(W.feff, i, F3<R>(W.force, i));
+    bool kin = W.type[i] == CHP_BODY_KINEMATIC;
+    W.ims[i] = kin ? (R)0 : W.inv_mass[i];
+    for (int k = 0; k < 9; k++) W.iiws[9 * i + k] = kin ? 0.f : W.iiw[9 * i + k];
+    S3<R>(W.vl, i, mk3(0.f, 0.f, 0.f));
+    S3<R>(W.wl, i, mk3(0.f, 0.f, 0.f));
+  }
+  __syncthreads();
+  // one pass per equation kind, each with a single row builder: the builders are not merged
+  // into one body
+  for (int pass = 0; pass < 4; pass++) {
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const chp_equation& e = eq[r];
+      const f3 ri = mk3(e.ri[0], e.ri[1], e.ri[2]), rj = mk3(e.rj[0], e.rj[1], e.rj[2]);
+      const f3 nn = mk3(e.n[0], e.n[1], e.n[2]);
+      const R sa_ = (R)e.a, sb_ = (R)e.b, se = (R)e.eps, mn = (R)e.min_force, mx = (R)e.max_force;
+      if (pass == 0 && e.kind == CHP_ROW_CONTACT)
+        contact_row(W, r, e.body_i, e.body_j, ri, rj, nn, (R)e.restitution, sa_, sb_, se, h, mn, mx);
+      else if (pass == 1 && e.kind == CHP_ROW_FRICTION)
+        friction_row(W, r, e.body_i, e.body_j, ri, rj, nn, sb_, se, h, mn, mx);
+      else if (pass == 2 && (e.kind == CHP_ROW_ROTATIONAL || e.kind == CHP_ROW_CONE))
+        rot_row(W, r, e.kind, e.body_i, e.body_j, ri, rj, (R)e.angle_cos, h, mn, mx, sa_, sb_, se);
+      else if (pass == 3 && e.kind == CHP_ROW_MOTOR)
+        motor_row(W, r, e.body_i, e.body_j, ri, rj, (R)e.target_velocity, h, mn, mx, sb_, se);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { W.counters[C_ROWS] = n; W.counters[C_STATUS] = 0; }
+  __syncthreads();
+}
+
+// end of GSSolver.solve: v += vlambda * linearFactor, w += wlambda * angularFactor
+// (gs_solver.dart:111-121)
+template <typename R> __device__ void st_solve_finish(const DevWorld<R>& W) {
+  if (W.counters[C_ROWS] == 0) return;
+  for (int i = threadIdx.x; i < W.nb; i += blockDim.x) {
+    f3 v = F3<R>(W.vel, i), w = F3<R>(W.avel, i);
+    S3<R>(W.vel, i, add3<R>(v, mul3<R>(F3<R>(W.vl, i), F3<R>(W.lin_factor, i))));
+    S3<R>(W.avel, i, add3<R>(w, mul3<R>(F3<R>(W.wl, i), F3<R>(W.ang_factor, i))));
+  }
 }
 
 // ---- Gauss-Seidel -------------------------------------------------------------------------------

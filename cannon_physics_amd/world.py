@@ -136,6 +136,50 @@ class Context:
                 rows_to_numpy(rows, int(nrows.value)), int(iters.value))
 
 
+    # ---- slots ----
+    def narrowphase(self, wid, pi, pj, dt, cap=1 << 18):
+        """Narrowphase slot: the equations the solver receives for these pairs (friction first)."""
+        n = len(pi)
+        api = (C.c_int32 * max(1, n))(*[int(x) for x in pi])
+        apj = (C.c_int32 * max(1, n))(*[int(x) for x in pj])
+        out = (A.Equation * cap)()
+        neq, nf, nc = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self.check(self.lib.chp_narrowphase(self.ptr, wid, api, apj, n, dt, out, cap, C.byref(neq), C.byref(nf),
+                                            C.byref(nc)))
+        return out, int(neq.value), int(nf.value), int(nc.value)
+
+    def solve(self, wid, eqs, n, dt):
+        """Solver slot: GSSolver.solve on the device state; returns (iterations, multipliers)."""
+        it = C.c_int32()
+        mult = (C.c_double * max(1, n))()
+        self.check(self.lib.chp_solve(self.ptr, wid, eqs, n, dt, C.byref(it), mult))
+        return int(it.value), np.frombuffer(mult, dtype=np.float64, count=n).copy()
+
+    # ---- serving ----
+    def serve_submit(self, recipe, dt, nsteps):
+        t = C.c_uint64()
+        self.check(self.lib.chp_serve_submit(self.ptr, recipe.ptr, dt, nsteps, C.byref(t)))
+        return int(t.value)
+
+    def serve_tick(self, max_steps):
+        r, q = C.c_uint32(), C.c_uint32()
+        self.check(self.lib.chp_serve_tick(self.ptr, max_steps, C.byref(r), C.byref(q)))
+        return int(r.value), int(q.value)
+
+    def serve_status(self, ticket):
+        st = A.RequestStatus()
+        self.check(self.lib.chp_serve_status(self.ptr, ticket, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in A.RequestStatus._fields_ if k != "reserved"}
+
+    def serve_result(self, ticket, n):
+        arr = (A.BodyState * max(1, n))()
+        self.check(self.lib.chp_serve_result(self.ptr, ticket, arr, n))
+        return arr
+
+    def serve_cancel(self, ticket):
+        self.check(self.lib.chp_serve_cancel(self.ptr, ticket))
+
+
 _default_ctx = {}
 
 

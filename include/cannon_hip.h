@@ -196,6 +196,24 @@ typedef struct chp_row {
   double b, inv_c, eps, min_force, max_force, lambda;
 } chp_row;
 
+/* one equation as the Narrowphase slot hands it to the Solver slot (ContactEquation,
+ * FrictionEquation, RotationalEquation, ConeEquation, RotationalMotorEquation; lib/equations/):
+ * the inputs of Equation.computeB, before B and 1/C exist */
+typedef struct chp_equation {
+  int32_t kind;                          /* CHP_ROW_* */
+  int32_t body_i, body_j;                /* Equation.bi / .bj, world-local */
+  int32_t reserved;
+  float ri[3], rj[3];                    /* contact / friction: ri, rj; rotational / cone: axisA (ni),
+                                            axisB (nj); motor: axisA, axisB */
+  float n[3];                            /* contact: ni; friction: t; unused otherwise */
+  float pad;
+  double min_force, max_force;
+  double a, b, eps;                      /* Equation.setSpookParams */
+  double restitution;                    /* ContactEquation.restitution */
+  double angle_cos;                      /* rotational / cone: cos(maxAngle) */
+  double target_velocity;                /* RotationalMotorEquation.targetVelocity */
+} chp_equation;
+
 /* ---- context ----------------------------------------------------------------------------- */
 typedef struct chp_ctx chp_ctx;
 
@@ -223,11 +241,12 @@ int chp_state_download(chp_ctx* ctx, uint32_t world_id, chp_body_state* out, uin
 int chp_state_upload(chp_ctx* ctx, uint32_t world_id, const chp_body_state* in, uint32_t n);
 
 /* ---- stepping ---------------------------------------------------------------------------- */
-/* Batched path: advances every listed world by nsteps steps of dt in one persistent launch
- * (one workgroup per world, the whole internalStep fused); state stays on the device.
- * Equivalent to calling World.step(dt) nsteps times on each world. */
+/* Batched path: advances every listed world by nsteps steps of dt, one launch per stage over all
+ * listed worlds per step; state stays on the device.  Equivalent to calling World.step(dt)
+ * nsteps times on each world (world_class.dart:392-431 at a fixed step). */
 int chp_step(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, double dt, int32_t nsteps);
-/* Plain path: one world, one launch per stage per step. Same results as chp_step. */
+/* Plain path: one world, one launch per stage per step and a host synchronisation per step.
+ * Same results as chp_step. */
 int chp_step_plain(chp_ctx* ctx, uint32_t world_id, double dt, int32_t nsteps);
 /* Asynchronous batched step on the context's stream; chp_sync waits for it. */
 int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, double dt,
@@ -235,8 +254,9 @@ int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, doubl
 int chp_sync(chp_ctx* ctx);
 /* device time (HIP events on the context stream) of the last chp_step / chp_step_plain call */
 int chp_last_step_ms(chp_ctx* ctx, double* ms);
-/* launch shape of the GS stage: workgroup size (64, 128 or 256; 0 keeps) and the largest body
- * count whose lambda velocities are kept in LDS (0 keeps) */
+/* launch shape of the level and GS stages: workgroup size (64, 128 or 256; 0 keeps).  A large
+ * world takes a workgroup, small worlds (<= 128 bodies) take one wave each.  lds_cap_bodies is
+ * accepted for compatibility and ignored. */
 int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int lds_cap_bodies);
 /* per-stage HIP events in chp_step (costs one event per stage kernel) */
 int chp_ctx_set_profiling(chp_ctx* ctx, int on);
@@ -265,6 +285,48 @@ int chp_broadphase(chp_ctx* ctx, uint32_t world_id, int32_t* pi, int32_t* pj, ui
 int chp_step_capture(chp_ctx* ctx, uint32_t world_id, double dt,
                      int32_t* pi, int32_t* pj, uint32_t pair_cap, uint32_t* npairs,
                      chp_row* rows, uint32_t row_cap, uint32_t* nrows, int32_t* iterations);
+
+/* Narrowphase.getContacts (narrow_phase.dart:634-721) for the given pairs on the world's current
+ * state, with World.internalStep's row assembly (world_class.dart:539-635): the equations the
+ * solver receives, friction equations first (2 per contact), then contact equations, in
+ * generation order; disabled and trigger contacts dropped, the body-material restitution
+ * override applied.  dt is World.dt (it enters the spook parameters). */
+int chp_narrowphase(chp_ctx* ctx, uint32_t world_id, const int32_t* pi, const int32_t* pj, uint32_t npairs,
+                    double dt, chp_equation* out, uint32_t cap, uint32_t* nequations, uint32_t* nfriction,
+                    uint32_t* ncontact);
+/* GSSolver.solve(dt, world) (gs_solver.dart:27-133) on the world's current device state for the
+ * given equations in solver order: B and 1/C from each equation's own spook parameters and the
+ * bodies' applied force as stored (the World adds gravity before calling its solver), the
+ * world's iterations and tolerance, then velocity += vlambda * linearFactor and
+ * angularVelocity += wlambda * angularFactor.  iterations receives GSSolver.solve's return
+ * value; multipliers (may be NULL) receive Equation.multiplier = lambda / dt. */
+int chp_solve(chp_ctx* ctx, uint32_t world_id, const chp_equation* eqs, uint32_t n, double dt,
+              int32_t* iterations, double* multipliers);
+
+/* ---- serving: continuous batching of requests --------------------------------------------- */
+/* A request is a world and a number of steps.  chp_serve_submit replays its constructors on the
+ * host and queues it; chp_serve_tick admits queued requests in arrival order while the
+ * context's HBM budget allows, advances every admitted request (one launch per stage over all
+ * of them, up to max_steps steps), retires the finished ones (their final state is kept on the
+ * host and their device memory freed) and returns.  Results do not depend on what else is in
+ * the batch. */
+enum { CHP_REQ_QUEUED = 0, CHP_REQ_RUNNING = 1, CHP_REQ_DONE = 2, CHP_REQ_FAILED = 3, CHP_REQ_CANCELLED = 4 };
+typedef struct chp_request_status {
+  int32_t state;                         /* CHP_REQ_* */
+  int32_t error;                         /* CHP_E_* when failed */
+  int32_t steps_done, steps_total;
+  uint32_t num_bodies;
+  uint32_t reserved;
+  double wait_ms;                        /* submit -> admission */
+  double latency_ms;                     /* submit -> completion (0 until done) */
+} chp_request_status;
+int chp_serve_submit(chp_ctx* ctx, const chp_scene* scene, double dt, int32_t nsteps, uint64_t* ticket);
+int chp_serve_tick(chp_ctx* ctx, int32_t max_steps, uint32_t* nrunning, uint32_t* nqueued);
+int chp_serve_status(chp_ctx* ctx, uint64_t ticket, chp_request_status* out);
+/* final state of a finished request; releases the ticket */
+int chp_serve_result(chp_ctx* ctx, uint64_t ticket, chp_body_state* out, uint32_t cap);
+/* drops a queued or running request (its device memory is freed at once) */
+int chp_serve_cancel(chp_ctx* ctx, uint64_t ticket);
 
 /* ---- setup helpers with reference semantics ---------------------------------------------- */
 /* Quaternion.setFromEuler(x, y, z) with order 'xyz' (quaternion.dart:191-232), FP32 result. */

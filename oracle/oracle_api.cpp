@@ -125,3 +125,62 @@ int orc_broadphase(void* wp, int* pi, int* pj, int cap) {
 }
 
 }  /* extern "C" */
+
+namespace orc {
+void narrowphase_equations(World& w, const std::vector<int>& pi, const std::vector<int>& pj, R dt,
+                           std::vector<Equation>& out, int& nf, int& nc);
+int solve_equations(World& w, std::vector<Equation>& eqs, R dt, std::vector<R>& multipliers);
+}
+
+extern "C" {
+
+/* Narrowphase slot: equations for the given pairs (friction then contact); returns the count */
+int orc_narrowphase(void* wp, const int* pi, const int* pj, int n, double dt, chp_equation* out, int cap, int* nf,
+                    int* nc) {
+  World& w = *(World*)wp;
+  std::vector<int> a(pi, pi + n), b(pj, pj + n);
+  std::vector<Equation> eqs;
+  orc::narrowphase_equations(w, a, b, (R)dt, eqs, *nf, *nc);
+  for (size_t k = 0; k < eqs.size() && (int)k < cap; k++) {
+    const Equation& e = eqs[k];
+    chp_equation& o = out[k];
+    memset(&o, 0, sizeof o);
+    o.kind = e.kind; o.body_i = e.bi->index; o.body_j = e.bj->index;
+    put3(o.ri, e.ri); put3(o.rj, e.rj);
+    put3(o.n, e.kind == CHP_ROW_FRICTION ? e.t : e.ni);
+    o.min_force = (double)e.minForce; o.max_force = (double)e.maxForce;
+    o.a = (double)e.a; o.b = (double)e.b; o.eps = (double)e.eps;
+    o.restitution = (double)e.restitution;
+  }
+  return (int)eqs.size();
+}
+
+/* Solver slot: GSSolver.solve on the current state for the given equations */
+int orc_solve(void* wp, const chp_equation* in, int n, double dt, double* multipliers) {
+  World& w = *(World*)wp;
+  std::vector<Equation> eqs(n);
+  for (int k = 0; k < n; k++) {
+    const chp_equation& e = in[k];
+    Equation& o = eqs[k];
+    o.kind = e.kind;
+    o.bi = w.bodies[e.body_i]; o.bj = w.bodies[e.body_j];
+    o.minForce = (R)e.min_force; o.maxForce = (R)e.max_force;
+    o.a = (R)e.a; o.b = (R)e.b; o.eps = (R)e.eps;
+    o.restitution = (R)e.restitution;
+    if (e.kind == CHP_ROW_CONTACT || e.kind == CHP_ROW_FRICTION) {
+      o.ri = get3(e.ri); o.rj = get3(e.rj);
+      if (e.kind == CHP_ROW_CONTACT) o.ni = get3(e.n); else o.t = get3(e.n);
+    } else {
+      o.axisA = get3(e.ri); o.axisB = get3(e.rj);
+      o.cosAngle = (R)e.angle_cos;
+      o.targetVelocity = (R)e.target_velocity;
+    }
+  }
+  std::vector<R> mult;
+  int it = orc::solve_equations(w, eqs, (R)dt, mult);
+  if (multipliers)
+    for (int k = 0; k < n; k++) multipliers[k] = (double)mult[k];
+  return it;
+}
+
+}  // extern "C"

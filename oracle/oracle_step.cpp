@@ -1167,4 +1167,44 @@ void internal_step(World& w, R dt) {
   w.stepnumber += 1;
 }
 
+/* The Narrowphase slot for given pairs (narrow_phase.dart:634-721) with World's assembly of
+ * what the solver receives (world_class.dart:539-562): friction equations, then contact
+ * equations, generation order, Solver.addEquation filter, restitution override. */
+void narrowphase_equations(World& w, const std::vector<int>& pi, const std::vector<int>& pj, R dt,
+                           std::vector<Equation>& out, int& nf, int& nc) {
+  w.dt = dt;
+  std::vector<Body*> p1, p2;
+  for (size_t k = 0; k < pi.size(); k++) { p1.push_back(w.bodies[pi[k]]); p2.push_back(w.bodies[pj[k]]); }
+  w.contacts.clear();
+  w.frictions.clear();
+  w.contacts.reserve(4 * p1.size() + 16);
+  NP np(w);
+  np.getContacts(p1, p2);
+  out.clear();
+  nf = nc = 0;
+  auto keep = [&](const Equation& e) { return e.enabled && !e.bi->isTrigger && !e.bj->isTrigger; };
+  for (auto& f : w.frictions)
+    if (keep(f)) { out.push_back(f); nf++; }
+  for (auto& c : w.contacts) {
+    if (c.bi->material >= 0 && c.bj->material >= 0) {
+      const MaterialDesc& ma = w.materials[c.bi->material];
+      const MaterialDesc& mb = w.materials[c.bj->material];
+      if (ma.restitution >= 0 && mb.restitution >= 0) c.restitution = ma.restitution * mb.restitution;
+    }
+    if (keep(c)) { out.push_back(c); nc++; }
+  }
+}
+
+/* The Solver slot: GSSolver.solve(dt, world) on the world's current state for given equations
+ * (gs_solver.dart:27-133); body forces are used as stored.  Returns the iterations. */
+int solve_equations(World& w, std::vector<Equation>& eqs, R dt, std::vector<R>& multipliers) {
+  std::vector<Equation*> ptrs;
+  for (auto& e : eqs) ptrs.push_back(&e);
+  std::vector<R> Bs, invCs, lambda;
+  int it = gs_solve(w, ptrs, dt, Bs, invCs, lambda);
+  multipliers.clear();
+  for (auto& e : eqs) multipliers.push_back(e.multiplier);
+  return it;
+}
+
 }  // namespace orc

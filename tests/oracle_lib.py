@@ -42,6 +42,9 @@ def lib(mode="f64"):
         "orc_capture_contacts": (C.c_int, [vp]),
         "orc_broadphase": (C.c_int, [vp, P(C.c_int), P(C.c_int), C.c_int]),
         "orc_capture_contact_points": (C.c_int, [vp, P(C.c_int), P(C.c_float), C.c_int]),
+        "orc_narrowphase": (C.c_int, [vp, P(C.c_int), P(C.c_int), C.c_int, C.c_double, P(A.Equation), C.c_int,
+                                      P(C.c_int), P(C.c_int)]),
+        "orc_solve": (C.c_int, [vp, P(A.Equation), C.c_int, C.c_double, P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -122,3 +125,19 @@ class OracleWorld:
 
     def sap_axis(self):
         return self.L.orc_sap_axis(self.ptr)
+
+    def narrowphase(self, pi, pj, dt, cap=1 << 18):
+        """Narrowphase slot on the current state: (equations, count, friction count, contact count)"""
+        n = len(pi)
+        api = (C.c_int * max(1, n))(*[int(x) for x in pi])
+        apj = (C.c_int * max(1, n))(*[int(x) for x in pj])
+        out = (A.Equation * cap)()
+        nf, nc = C.c_int(), C.c_int()
+        k = self.L.orc_narrowphase(self.ptr, api, apj, n, dt, out, cap, C.byref(nf), C.byref(nc))
+        return out, k, nf.value, nc.value
+
+    def solve(self, eqs, n, dt):
+        """Solver slot on the current state: (iterations, multipliers)"""
+        mult = (C.c_double * max(1, n))()
+        it = self.L.orc_solve(self.ptr, eqs, n, dt, mult)
+        return it, np.frombuffer(mult, dtype=np.float64, count=n).copy()
