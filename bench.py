@@ -114,6 +114,72 @@ def stats_totals(ctx, wids):
     return tot
 
 
+def serve_bench(ntemplates, nrequests, budget_gib, tick_steps, seed=5, max_submit_s=60.0):
+    """Serving: a Poisson stream of requests (worlds of log-uniform [16, 2048] bodies over a
+    64x64 heightfield, K ~ U{60..600} steps at dt = 1/60) through chp_serve_*.  The arrival
+    rate is set from Little's law so that the context's HBM budget is about 70% occupied.
+    Request worlds are drawn from ntemplates pre-built recipes."""
+    import numpy as np
+    from cannon_physics_amd import scenes
+    from cannon_physics_amd.world import Context
+    t0 = time.perf_counter()
+    temps = []
+    for i in range(ntemplates):
+        w = scenes.c5_terrain_world(seed * 1000 + i)
+        temps.append((w.recipe(), len(w.bodies) - 1))
+    log("serving: %d request templates built in %.1fs" % (ntemplates, time.perf_counter() - t0))
+    budget = int(budget_gib * (1 << 30))
+    ctx = Context(device=0, hbm_budget=budget)
+    # bytes per template, for the arrival rate
+    sizes = []
+    for r, _ in temps:
+        wid = ctx.create_world(r)
+        sizes.append(ctx.bytes_in_use())
+        ctx.destroy_world(wid)
+    rng = np.random.default_rng(seed)
+    K = rng.integers(60, 601, size=nrequests)
+    pick = rng.integers(0, ntemplates, size=nrequests)
+    mean_ticks = float(np.mean(K)) / tick_steps
+    rate = 0.7 * budget / (float(np.mean(sizes)) * mean_ticks)      # arrivals per tick
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nrequests))
+    tickets, submitted, tick, occ = [], 0, 0, []
+    t0 = time.perf_counter()
+    last = t0
+    while True:
+        now = time.perf_counter()
+        if now - last > 2.0:
+            log("serving: tick %d, %d submitted, %.0f%% of the budget in use" % (tick, submitted, 100 * occ[-1]))
+            last = now
+        if now - t0 > max_submit_s:
+            nrequests = submitted        # stop the arrival stream; drain what is in
+        while submitted < nrequests and arrivals[submitted] <= tick:
+            r, nb = temps[pick[submitted]]
+            tickets.append((ctx.serve_submit(r, 1 / 60, int(K[submitted])), nb, int(K[submitted])))
+            submitted += 1
+        running, queued = ctx.serve_tick(tick_steps)
+        occ.append(ctx.bytes_in_use() / budget)
+        tick += 1
+        if submitted == nrequests and running == 0 and queued == 0:
+            break
+    wall = time.perf_counter() - t0
+    lat, waits, bsteps = [], [], 0
+    for t, nb, k in tickets:
+        st = ctx.serve_status(t)
+        if st["state"] == 2:
+            lat.append(st["latency_ms"])
+            waits.append(st["wait_ms"])
+            bsteps += nb * k
+    ctx.close()
+    lat = np.array(lat)
+    return {"requests": submitted, "completed": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_wait_ms": float(np.mean(waits)),
+            "body_steps_per_s": bsteps / wall, "wall_s": wall, "mean_occupancy": float(np.mean(occ)),
+            "hbm_budget_gib": budget_gib, "steps_per_tick": tick_steps,
+            "config": "Poisson arrivals at %.2f requests per tick, %d request templates (log-uniform [16,2048] "
+                      "bodies: spheres, boxes, 8-16 vertex hulls over a 64x64 heightfield), K ~ U{60..600} "
+                      "steps, dt 1/60, FP32" % (rate, ntemplates)}
+
+
 def cpu_baseline(recs, dyn, dt, warm, stride, target_s=15.0):
     """The oracle (reference semantics, FP64 arithmetic) on every stride-th world, one
     std::thread per world slice, on the host cores; the number of timed steps is chosen so that
@@ -153,6 +219,10 @@ def main():
     ap.add_argument("--solve-block", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-templates", type=int, default=12)
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-budget-gib", type=float, default=8.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size once on the device (the "
                          "calibration kernel for HBM byte counters)")
@@ -220,6 +290,12 @@ def main():
                  "sample": "%d worlds (every %d-th), 5 steps each, one launch per stage per step "
                            "and a host synchronisation per step" % (len(sample), stride)}
 
+    serving = None
+    if not args.no_serving:
+        ctx.close()
+        serving = serve_bench(args.serve_templates, args.serve_requests, args.serve_budget_gib, 8)
+        log("serving: " + json.dumps(serving))
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(recs, dyn, dt, 5, 1)
@@ -248,6 +324,7 @@ def main():
         "dominant_kernel": dom,
         "stage_ms_per_step": {k: v / K for k, v in stages.items()},
         "plain_gpu": plain,
+        "serving": serving,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
     }

@@ -41,14 +41,44 @@ __global__ __launch_bounds__(256) void k_bp(const DevWorld<R>* __restrict__ worl
   if (ok) st_filter(W);
 }
 
-// narrowphase: one thread per pair
+// narrowphase: per-class pair counts per world, class-major offsets, placement in the queue,
+// then one pair per thread over the thread classes and one pair per wave over the wave class,
+// each grid-wide over the whole batch.  qcls: NP_NCLS ints per listed world; bounds:
+// NP_NCLS + 1 ints.
+template <typename R>
+__global__ __launch_bounds__(64) void k_narrow_count(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                     int* __restrict__ qcls) {
+  __shared__ int sh[64];
+  int* cnt = qcls + (size_t)blockIdx.x * NP_NCLS;
+  const DevWorld<R>& W = worlds[list[blockIdx.x]];
+  if (block_bcast(W.counters[C_STATUS], sh) != 0) {
+    if (threadIdx.x < NP_NCLS) cnt[threadIdx.x] = 0;
+    return;
+  }
+  st_narrow_count(W, cnt, sh);
+}
+__global__ __launch_bounds__(1024) void k_narrow_scan(int* __restrict__ qcls, int n, int* __restrict__ bounds) {
+  __shared__ int sh[1024];
+  st_narrow_scan(qcls, n, bounds, sh);
+}
 template <typename R>
 __global__ __launch_bounds__(64) void k_narrow(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                               double dt) {
+                                               const int* __restrict__ qcls, int2* __restrict__ q) {
   __shared__ int sh[64];
   CHP_WORLD_PROLOGUE
-  st_narrow(W, (R)dt);
+  st_narrow_place(W, list[blockIdx.x], qcls + (size_t)blockIdx.x * NP_NCLS, q, sh);
 }
+template <typename R>
+__global__ __launch_bounds__(64) void k_narrow_thread(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
+                                                      const int* __restrict__ bounds, double dt) {
+  st_narrow_thread(worlds, q, bounds[NP_WAVE], (R)dt);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_narrow_wave(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
+                                                     const int* __restrict__ bounds, double dt) {
+  st_narrow_wave(worlds, q + bounds[NP_WAVE], bounds[NP_NCLS] - bounds[NP_WAVE], (R)dt);
+}
+
 
 // equations: friction, contact and constraint rows with B and 1/C
 template <typename R>
@@ -315,6 +345,10 @@ struct chp_ctx {
   std::vector<hipEvent_t> sev;               // 7 events per step
   std::vector<double> stage_ms;              // accumulated per stage over the last chp_step
   int solve_block = 256;   // workgroup of k_levels / k_gs: one large world or blockDim/64 small ones
+  int2* d_q = nullptr;           // narrowphase queue of a step (world, pair), sorted by resolver class
+  size_t d_q_cap = 0;
+  int* d_qcls = nullptr;         // per-world class counts / offsets, then the class bounds
+  size_t d_qcls_cap = 0;
   chp_equation* d_eq = nullptr;  // staging of the narrowphase / solver slots
   size_t d_eq_cap = 0;
   // serving
@@ -392,15 +426,15 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   D.nhf = (int)H.hf_nx.size();
   // record offsets; pointers are patched once the slab exists
   std::vector<std::pair<void**, size_t>> fix;
-#define ARR(field, vec) fix.emplace_back((void**)&D.field, L.add(vec))
-#define SCR(field, T, n) fix.emplace_back((void**)&D.field, L.add<T>(n))
+#define ARR(field, vec) fix.emplace_back((void**)(void*)&D.field, L.add(vec))
+#define SCR(field, T, n) fix.emplace_back((void**)(void*)&D.field, L.add<T>(n))
   ARR(pos, H.pos); ARR(quat, H.quat); ARR(vel, H.vel); ARR(avel, H.avel); ARR(force, H.force); ARR(torque, H.torque);
   ARR(iiw, H.iiw); ARR(inv_inertia, H.inv_inertia); ARR(lin_factor, H.lin_factor); ARR(ang_factor, H.ang_factor);
   ARR(aabb_lo, H.aabb_lo); ARR(aabb_hi, H.aabb_hi);
   ARR(mass, H.mass); ARR(inv_mass, H.inv_mass); ARR(brad, H.brad);
   size_t o_ld = L.add<R>(nb), o_ad = L.add<R>(nb);
-  fix.emplace_back((void**)&D.ldamp, o_ld);
-  fix.emplace_back((void**)&D.adamp, o_ad);
+  fix.emplace_back((void**)(void*)&D.ldamp, o_ld);
+  fix.emplace_back((void**)(void*)&D.adamp, o_ad);
   ARR(type, H.type); ARR(flags, H.flags); ARR(group, H.group); ARR(mask, H.mask); ARR(material, H.material);
   ARR(shape_first, H.shape_first); ARR(shape_count, H.shape_count); ARR(order, H.order);
   ARR(si_type, H.si_type); ARR(si_hull, H.si_hull); ARR(si_hf, H.si_hf); ARR(si_response, H.si_response);
@@ -429,12 +463,12 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   SCR(rows, RowRec<R>, row_cap);
   SCR(level, int, row_cap); SCR(sched, int, row_cap); SCR(lvl_start, int, row_cap + 2);
   SCR(row_ab, int, 2 * row_cap); SCR(rpos, int, row_cap); SCR(absdl, R, row_cap + 1); SCR(glam, R, row_cap + 1);
-  fix.emplace_back((void**)&D.gsr, L.add<R>((size_t)4 * GS_GROUPS * row_cap));
+  fix.emplace_back((void**)(void*)&D.gsr, L.add<R>((size_t)4 * GS_GROUPS * row_cap));
   SCR(vw, float, 8 * nb);
   SCR(last_level, int, std::max(nb, row_cap + 1));
   SCR(sort_keys, unsigned long long, np2);
   size_t o_cnt = L.add<int>(C_COUNT);
-  fix.emplace_back((void**)&D.counters, o_cnt);
+  fix.emplace_back((void**)(void*)&D.counters, o_cnt);
 #undef ARR
 #undef SCR
   if (ctx->in_use + L.size > ctx->budget)
@@ -599,6 +633,29 @@ static int check_status(chp_ctx* ctx, const uint32_t* ids, uint32_t n) {
   return CHP_OK;
 }
 
+// the narrowphase queue holds one entry per pair of the listed worlds (at most their pair
+// capacities); qcls holds the per-world class counts and the class bounds
+static int ensure_queue(chp_ctx* ctx, const uint32_t* ids, uint32_t n) {
+  size_t need = 1;
+  for (uint32_t k = 0; k < n; k++) need += (size_t)ctx->worlds[ids[k]].pair_cap;
+  if (need > ((size_t)1 << 31) - 1) return fail(ctx, CHP_E_CAPACITY, "more than 2^31 pairs in one batch");
+  size_t ncls = (size_t)n * NP_NCLS + NP_NCLS + 1;
+  if (ctx->d_qcls_cap < ncls) {
+    if (ctx->d_qcls) (void)hipFree(ctx->d_qcls);
+    ctx->d_qcls = nullptr;
+    ctx->d_qcls_cap = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->d_qcls, ncls * sizeof(int)));
+    ctx->d_qcls_cap = ncls;
+  }
+  if (ctx->d_q_cap >= need) return CHP_OK;
+  if (ctx->d_q) (void)hipFree(ctx->d_q);
+  ctx->d_q = nullptr;
+  ctx->d_q_cap = 0;
+  HIPCHK(ctx, hipMalloc(&ctx->d_q, need * sizeof(int2)));
+  ctx->d_q_cap = need;
+  return CHP_OK;
+}
+
 enum { K_BP = 0, K_NARROW, K_ROWS, K_LEVELS, K_GS, K_POST, K_NSTAGE };
 static const char* kStageNames[K_NSTAGE] = {"k_bp", "k_narrow", "k_rows", "k_levels", "k_gs", "k_post"};
 
@@ -613,7 +670,17 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
   if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
   k_bp<R><<<n, 256, 0, st>>>(dw, d_list);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
-  k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, dt);
+  {
+    const int qcap = (int)ctx->d_q_cap;
+    int* bounds = ctx->d_qcls + (size_t)n * NP_NCLS;
+    k_narrow_count<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls);
+    k_narrow_scan<<<1, 1024, 0, st>>>(ctx->d_qcls, n, bounds);
+    k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls, ctx->d_q);
+    const int threads = std::max(64, std::min(qcap, 256 * 4 * 64));
+    k_narrow_thread<R><<<(threads + 63) / 64, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    const int waves = std::max(1, std::min(qcap, 4096));
+    k_narrow_wave<R><<<(waves + 3) / 4, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
+  }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
   k_rows<R><<<n, 128, 0, st>>>(dw, d_list, dt);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
@@ -632,6 +699,8 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
 template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt, int nsteps,
                                            bool wait, std::vector<int>* status) {
   int r = prepare<R>(ctx, ids, n, dt);
+  if (r != CHP_OK) return r;
+  r = ensure_queue(ctx, ids, n);
   if (r != CHP_OK) return r;
   if (ctx->d_list_cap < n) {
     if (ctx->d_list) (void)hipFree(ctx->d_list);
@@ -685,6 +754,8 @@ template <typename R> static int step_t(chp_ctx* ctx, const uint32_t* ids, uint3
 
 template <typename R> static int step_plain_t(chp_ctx* ctx, uint32_t id, double dt, int nsteps) {
   int r = prepare<R>(ctx, &id, 1, dt);
+  if (r != CHP_OK) return r;
+  r = ensure_queue(ctx, &id, 1);
   if (r != CHP_OK) return r;
   if (ctx->d_list_cap < 1) {
     HIPCHK(ctx, hipMalloc(&ctx->d_list, sizeof(int)));
@@ -1110,6 +1181,8 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   if (ctx->d_status) (void)hipFree(ctx->d_status);
   if (ctx->d_cptrs) (void)hipFree(ctx->d_cptrs);
   if (ctx->d_eq) (void)hipFree(ctx->d_eq);
+  if (ctx->d_q) (void)hipFree(ctx->d_q);
+  if (ctx->d_qcls) (void)hipFree(ctx->d_qcls);
   for (hipEvent_t e : ctx->sev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);

@@ -34,25 +34,32 @@ template <typename R> CHP_HD HullView<R> hull_view(const DevWorld<R>& W, int h) 
 
 struct SRef { int resp; int mat; };   // Shape.collisionResponse / Shape.material
 
-// where resolvers put contacts
-template <typename R> struct Sink {
+// where resolvers put contacts.  WAVE: every lane of a wave computes the same contacts and
+// lane 0 stores them.
+template <typename R, bool WAVE = false> struct Sink {
   ContactRec<R>* buf;  // local array or global slab
   int cap;             // writes beyond cap are counted, not stored
   int n;
   int overflow;
   CHP_HD void push(const ContactRec<R>& c) {
-    if (n < cap) buf[n] = c; else overflow = 1;
+    if (n < cap) {
+      if (!WAVE || (threadIdx.x & 63) == 0) buf[n] = c;
+    } else {
+      overflow = 1;
+    }
     n++;
   }
 };
 
-template <typename R> struct NPCtx {
+// A resolver context.  WAVE: a whole wave runs one pair, every lane on the same data (so every
+// branch is uniform), and the separating-axis search splits its axes over the lanes.
+template <typename R, bool WAVE = false> struct NPCtx {
   const DevWorld<R>& W;
   R dt;
   int cm;                 // currentContactMaterial
-  Sink<R>* sink;
+  Sink<R, WAVE>* sink;
   int status;
-  CHP_HD NPCtx(const DevWorld<R>& w, R dt_, Sink<R>* s) : W(w), dt(dt_), cm(0), sink(s), status(0) {}
+  CHP_HD NPCtx(const DevWorld<R>& w, R dt_, Sink<R, WAVE>* s) : W(w), dt(dt_), cm(0), sink(s), status(0) {}
 
   // Narrowphase.createContactEquation (narrow_phase.dart:492-528) +
   // createFrictionEquationsFromContact (:530-586) + the World's body-material restitution
@@ -337,6 +344,7 @@ template <typename R> struct NPCtx {
   // findSeparatingAxis (:232-356); face0_only: faceListA = [0]
   CHP_HD static bool findSepAxis(const HullView<R>& A, const HullView<R>& B, f3 pa, f4 qa, f3 pb, f4 qb, bool face0_only,
                                  f3& target) {
+    if constexpr (WAVE) return findSepAxisWave(A, B, pa, qa, pb, qb, face0_only, target);
     R dmin = (R)__builtin_huge_val();
     const f3 zero = mk3(0.f, 0.f, 0.f);
     f3 loA = point_to_local<R>(pa, qa, zero), loB = point_to_local<R>(pb, qb, zero);
@@ -370,6 +378,56 @@ template <typename R> struct NPCtx {
         }
       }
     }
+    f3 dc = sub3<R>(pb, pa);
+    if (dot3<R>(dc, target) > (R)0) target = neg3(target);
+    return true;
+  }
+  // The same search over the same axes in the same order, the axes dealt to the lanes of a
+  // wave.  The sequential search returns "separated" at the first separating axis, else the
+  // first axis of least depth (strict <): here any lane's separating axis answers the first, a
+  // (depth, axis index) minimum over the lanes the second.
+  __device__ static bool findSepAxisWave(const HullView<R>& A, const HullView<R>& B, f3 pa, f4 qa, f3 pb, f4 qb,
+                                         bool face0_only, f3& target) {
+    const int lane = threadIdx.x & 63;
+    const f3 zero = mk3(0.f, 0.f, 0.f);
+    f3 loA = point_to_local<R>(pa, qa, zero), loB = point_to_local<R>(pb, qb, zero);
+    const int nA = A.axes ? (face0_only ? 1 : A.nf) : 0, nB = B.axes ? B.nf : 0;
+    const int total = nA + nB + A.ne * B.ne;
+    R best = (R)__builtin_huge_val();
+    int bestk = 0x7fffffff;
+    f3 bax = zero;
+    bool sep = false;
+    for (int k = lane; k < total; k += 64) {
+      f3 axis;
+      bool valid = true;
+      if (k < nA) {
+        axis = qrot<R>(qa, A.fn[k]);
+      } else if (k < nA + nB) {
+        axis = qrot<R>(qb, B.fn[k - nA]);
+      } else {
+        int e = k - nA - nB, e0 = e / B.ne, e1 = e - e0 * B.ne;
+        f3 cr = cross3<R>(qrot<R>(qa, A.e[e0]), qrot<R>(qb, B.e[e1]));
+        valid = !almost_zero3<R>(cr);
+        axis = valid ? normalize3<R>(cr) : zero;
+      }
+      if (valid) {
+        R d;
+        if (!testSepAxis(A, axis, B, loA, qa, loB, qb, d)) sep = true;
+        else if (d < best) { best = d; bestk = k; bax = axis; }
+      }
+    }
+    if (__any(sep)) return false;
+    // lexicographic (depth, axis index) minimum over the wave
+    int win = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+      R ob = __shfl_xor(best, off, 64);
+      int ok = __shfl_xor(bestk, off, 64), ow = __shfl_xor(win, off, 64);
+      if (ob < best || (ob == best && ok < bestk)) { best = ob; bestk = ok; win = ow; }
+    }
+    win = __builtin_amdgcn_readfirstlane(win);
+    target = mk3(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(bax.x), win)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bax.y), win)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bax.z), win)));
     f3 dc = sub3<R>(pb, pa);
     if (dot3<R>(dc, target) > (R)0) target = neg3(target);
     return true;

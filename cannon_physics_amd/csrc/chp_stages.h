@@ -256,42 +256,182 @@ template <typename R> __device__ void st_filter(const DevWorld<R>& W) {
 }
 
 // ---- narrowphase ------------------------------------------------------------------------------
-template <typename R> __device__ void st_narrow(const DevWorld<R>& W, R dt) {
-  int P = W.counters[C_PAIRS];
-  for (int p = threadIdx.x; p < P; p += blockDim.x) {
-    int bi = W.pair_i[p], bj = W.pair_j[p];
-    int n = 0, base = 0, nf = 0, nc = 0;
-    if (bi >= 0) {
-      ContactRec<R> loc[CHP_KMAX];
-      Sink<R> s{loc, CHP_KMAX, 0, 0};
-      NPCtx<R> ctx(W, dt, &s);
-      ctx.pair(bi, bj);
-      if (ctx.status) ws_fail(W.counters, ctx.status);
-      n = s.n;
-      if (n > 0) {
-        base = atomicAdd(&W.counters[C_STAGING], n);
-        if (base + n > W.contact_cap) {
-          ws_fail(W.counters, WS_CONTACTS);
-          n = 0;
-        } else if (n <= CHP_KMAX) {
-          for (int k = 0; k < n; k++) W.contacts[base + k] = loc[k];
-        } else {  // more contacts than the local buffer: run the pair again into the slab
-          Sink<R> s2{W.contacts + base, n, 0, 0};
-          NPCtx<R> ctx2(W, dt, &s2);
-          ctx2.pair(bi, bj);
-        }
-        for (int k = 0; k < n; k++) {
-          int fl = W.contacts[base + k].flags;
-          if (fl & 1) { nc++; if (fl & 2) nf += 2; }
-        }
+// Pairs whose separating-axis search is large (more than WAVE_AXES axes: convex hulls without
+// unique axes, anything against heightfield pillars) run one pair per wave, the axes over the
+// lanes; every other pair runs one pair per thread.
+constexpr int WAVE_AXES = 96;
+template <typename R> __device__ __forceinline__ bool wave_pair(const DevWorld<R>& W, int bi, int bj) {
+  for (int a = 0; a < W.shape_count[bi]; a++)
+    for (int b = 0; b < W.shape_count[bj]; b++) {
+      int sa = W.shape_first[bi] + a, sb = W.shape_first[bj] + b;
+      int ta = W.si_type[sa], tb = W.si_type[sb];
+      bool ca = ta == CHP_SHAPE_BOX || ta == CHP_SHAPE_CONVEX, cb = tb == CHP_SHAPE_BOX || tb == CHP_SHAPE_CONVEX;
+      if ((ca && tb == CHP_SHAPE_HEIGHTFIELD) || (cb && ta == CHP_SHAPE_HEIGHTFIELD)) return true;
+      if (ca && cb) {
+        int ha = W.si_hull[sa], hb = W.si_hull[sb];
+        int axes = (W.h_axes[ha] ? W.h_fcount[ha] : 0) + (W.h_axes[hb] ? W.h_fcount[hb] : 0) +
+                   W.h_ecount[ha] * W.h_ecount[hb];
+        if (axes > WAVE_AXES) return true;
       }
     }
-    W.pc_cnt[p] = n;
-    W.pc_base[p] = base;
-    W.pf_cnt[p] = nf;
-    W.pc_off[p] = nc;
+  return false;
+}
+
+// records a pair's contacts: staging slot, per-pair counts (pc_cnt / pc_base / pf_cnt / pc_off)
+template <typename R> __device__ __forceinline__ void pair_counts(const DevWorld<R>& W, int p, int n, int base) {
+  int nf = 0, nc = 0;
+  for (int k = 0; k < n; k++) {
+    int fl = W.contacts[base + k].flags;
+    if (fl & 1) { nc++; if (fl & 2) nf += 2; }
+  }
+  W.pc_cnt[p] = n;
+  W.pc_base[p] = base;
+  W.pf_cnt[p] = nf;
+  W.pc_off[p] = nc;
+}
+
+// one pair on one thread: contacts into the staging slab, per-pair counts
+template <typename R> __device__ __forceinline__ void narrow_pair(const DevWorld<R>& W, int p, R dt) {
+  int bi = W.pair_i[p], bj = W.pair_j[p];
+  int n = 0, base = 0;
+  if (bi >= 0) {
+    ContactRec<R> loc[CHP_KMAX];
+    Sink<R> s{loc, CHP_KMAX, 0, 0};
+    NPCtx<R> ctx(W, dt, &s);
+    ctx.pair(bi, bj);
+    if (ctx.status) ws_fail(W.counters, ctx.status);
+    n = s.n;
+    if (n > 0) {
+      base = atomicAdd(&W.counters[C_STAGING], n);
+      if (base + n > W.contact_cap) {
+        ws_fail(W.counters, WS_CONTACTS);
+        n = 0;
+      } else if (n <= CHP_KMAX) {
+        for (int k = 0; k < n; k++) W.contacts[base + k] = loc[k];
+      } else {  // more contacts than the local buffer: run the pair again into the slab
+        Sink<R> s2{W.contacts + base, n, 0, 0};
+        NPCtx<R> ctx2(W, dt, &s2);
+        ctx2.pair(bi, bj);
+      }
+    }
+  }
+  pair_counts(W, p, n, base);
+}
+
+// narrowphase of one world on one workgroup, one thread per pair (the slot entry point)
+template <typename R> __device__ void st_narrow(const DevWorld<R>& W, R dt) {
+  int P = W.counters[C_PAIRS];
+  for (int p = threadIdx.x; p < P; p += blockDim.x) narrow_pair(W, p, dt);
+  __syncthreads();
+}
+
+// The batched narrowphase sorts the pairs of every listed world by resolver class, so that a
+// wave of the thread kernel runs one resolver on neighbouring data.  Classes 0..NP_WAVE-1 run one
+// pair per thread, class NP_WAVE one pair per wave.
+enum { NP_SS = 0, NP_SP, NP_SB, NP_SC, NP_SH, NP_PC, NP_CC, NP_OTHER, NP_WAVE, NP_NCLS };
+template <typename R> __device__ __forceinline__ int pair_class(const DevWorld<R>& W, int bi, int bj) {
+  if (W.shape_count[bi] != 1 || W.shape_count[bj] != 1) return wave_pair(W, bi, bj) ? NP_WAVE : NP_OTHER;
+  int ta = W.si_type[W.shape_first[bi]], tb = W.si_type[W.shape_first[bj]];
+  int lo = min(ta, tb), hi = max(ta, tb);
+  if (lo == CHP_SHAPE_SPHERE) {
+    if (hi == CHP_SHAPE_SPHERE) return NP_SS;
+    if (hi == CHP_SHAPE_PLANE) return NP_SP;
+    if (hi == CHP_SHAPE_BOX) return NP_SB;
+    if (hi == CHP_SHAPE_CONVEX) return NP_SC;
+    if (hi == CHP_SHAPE_HEIGHTFIELD) return NP_SH;
+    return NP_OTHER;
+  }
+  if (lo == CHP_SHAPE_PLANE && (hi == CHP_SHAPE_BOX || hi == CHP_SHAPE_CONVEX)) return NP_PC;
+  if (lo == CHP_SHAPE_BOX || lo == CHP_SHAPE_CONVEX) {
+    if (hi == CHP_SHAPE_HEIGHTFIELD) return NP_WAVE;
+    if (hi == CHP_SHAPE_BOX || hi == CHP_SHAPE_CONVEX) return wave_pair(W, bi, bj) ? NP_WAVE : NP_CC;
+  }
+  return NP_OTHER;
+}
+
+// pass 1: per-class pair counts of the group's world (cnt: NP_NCLS ints of the group)
+template <typename R> __device__ void st_narrow_count(const DevWorld<R>& W, int* cnt, int* sh) {
+  int P = W.counters[C_PAIRS];
+  if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = 0;
+  __syncthreads();
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    int bi = W.pair_i[p], bj = W.pair_j[p];
+    if (bi < 0) { pair_counts(W, p, 0, 0); continue; }
+    atomicAdd(&sh[pair_class(W, bi, bj)], 1);
   }
   __syncthreads();
+  if (threadIdx.x < NP_NCLS) cnt[threadIdx.x] = sh[threadIdx.x];
+}
+
+// pass 2: the world's pairs into the queue at its offsets (off: NP_NCLS ints of the group)
+template <typename R> __device__ void st_narrow_place(const DevWorld<R>& W, int wid, const int* off, int2* q, int* sh) {
+  int P = W.counters[C_PAIRS];
+  if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = off[threadIdx.x];
+  __syncthreads();
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    int bi = W.pair_i[p], bj = W.pair_j[p];
+    if (bi < 0) continue;
+    q[atomicAdd(&sh[pair_class(W, bi, bj)], 1)] = make_int2(wid, p);
+  }
+}
+
+// between the passes: offsets class-major over the listed worlds (cnt[w * NP_NCLS + c] becomes
+// the offset; bounds[c] = first entry of class c, bounds[NP_NCLS] = total).  One workgroup.
+__device__ inline void st_narrow_scan(int* cnt, int n, int* bounds, int* sh) {
+  int carry = 0;
+  for (int c = 0; c < NP_NCLS; c++) {
+    if (threadIdx.x == 0) bounds[c] = carry;
+    for (int base = 0; base < n; base += blockDim.x) {
+      int w = base + threadIdx.x;
+      int v = w < n ? cnt[w * NP_NCLS + c] : 0, tot;
+      int ex = block_exscan(v, sh, &tot);
+      if (w < n) cnt[w * NP_NCLS + c] = carry + ex;
+      carry += tot;
+    }
+  }
+  if (threadIdx.x == 0) bounds[NP_NCLS] = carry;
+}
+
+// the thread queue, one pair per thread (grid-stride)
+template <typename R> __device__ void st_narrow_thread(const DevWorld<R>* worlds, const int2* q, int cnt, R dt) {
+  const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x), ng = (int)(gridDim.x * blockDim.x);
+  for (int i = g; i < cnt; i += ng) {
+    const int2 e = q[i];
+    narrow_pair(worlds[e.x], e.y, dt);
+  }
+}
+
+// the queued separating-axis pairs, one wave per pair (grid-stride over the queue)
+template <typename R> __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, R dt) {
+  const int lane = threadIdx.x & 63;
+  const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
+  for (int i = gw; i < cnt; i += nw) {
+    const int2 q = queue[i];
+    const DevWorld<R>& W = worlds[q.x];
+    const int p = q.y;
+    const int bi = W.pair_i[p], bj = W.pair_j[p];
+    ContactRec<R> loc[CHP_KMAX];
+    Sink<R, true> s{loc, CHP_KMAX, 0, 0};
+    NPCtx<R, true> ctx(W, dt, &s);
+    ctx.pair(bi, bj);
+    int n = s.n, base = 0;
+    if (lane == 0) {
+      if (ctx.status) ws_fail(W.counters, ctx.status);
+      if (n > 0) base = atomicAdd(&W.counters[C_STAGING], n);
+    }
+    base = __shfl(base, 0, 64);
+    if (n > 0 && base + n > W.contact_cap) {
+      if (lane == 0) ws_fail(W.counters, WS_CONTACTS);
+      n = 0;
+    } else if (n > CHP_KMAX) {  // more contacts than the local buffer: run the pair again into the slab
+      Sink<R, true> s2{W.contacts + base, n, 0, 0};
+      NPCtx<R, true> ctx2(W, dt, &s2);
+      ctx2.pair(bi, bj);
+    } else if (lane == 0) {
+      for (int k = 0; k < n; k++) W.contacts[base + k] = loc[k];
+    }
+    if (lane == 0) pair_counts(W, p, n, base);
+  }
 }
 
 // ---- equations ----------------------------------------------------------------------------------
