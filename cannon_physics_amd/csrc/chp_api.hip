@@ -29,15 +29,29 @@ using namespace chp;
   const DevWorld<R>& W = worlds[list[blockIdx.x]];                  \
   if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
 
-// gravity / solve mass / AABBs, broadphase and the collideConnected filter
+// gravity / solve mass / AABBs, broadphase and the collideConnected filter.  Dynamic LDS holds
+// the naive broadphase's body data of worlds with at most lds_nb bodies.
 template <typename R>
-__global__ __launch_bounds__(256) void k_bp(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list) {
+__global__ __launch_bounds__(256) void k_bp(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                            int lds_nb) {
   __shared__ int sh[256];
+  extern __shared__ double lds_d[];
   CHP_WORLD_PROLOGUE
   if (threadIdx.x == 0) W.counters[C_STAGING] = 0;
   st_prestep(W);
   __syncthreads();
-  bool ok = W.broadphase == CHP_BROADPHASE_SAP ? st_bp_sap(W, sh) : st_bp_naive(W, sh);
+  bool ok;
+  if (W.broadphase == CHP_BROADPHASE_SAP) {
+    ok = st_bp_sap(W, sh);
+  } else {
+    BpLds<R> L;
+    if (W.nb <= lds_nb) {
+      L.carve(lds_d, lds_nb);
+      ok = st_bp_naive<R, true>(W, sh, L);
+    } else {
+      ok = st_bp_naive<R, false>(W, sh, L);
+    }
+  }
   if (ok) st_filter(W);
 }
 
@@ -189,7 +203,8 @@ __global__ __launch_bounds__(CHP_BLOCK) void k_broadphase(const DevWorld<R>* __r
     __syncthreads();
     st_bp_sap(W, sh);
   } else {
-    st_bp_naive(W, sh);
+    BpLds<R> L;
+    st_bp_naive<R, false>(W, sh, L);
   }
 }
 
@@ -349,6 +364,7 @@ struct chp_ctx {
   size_t d_q_cap = 0;
   int* d_qcls = nullptr;         // per-world class counts / offsets, then the class bounds
   size_t d_qcls_cap = 0;
+  size_t max_lds = 65536;        // LDS bytes a workgroup may allocate on the device
   chp_equation* d_eq = nullptr;  // staging of the narrowphase / solver slots
   size_t d_eq_cap = 0;
   // serving
@@ -661,14 +677,19 @@ static const char* kStageNames[K_NSTAGE] = {"k_bp", "k_narrow", "k_rows", "k_lev
 
 // launches the stage kernels of one step for the worlds in d_list[0..n); ev (optional) gets
 // K_NSTAGE + 1 events around them
-template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, int n, int nbig, double dt,
+template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, int n, int nbig, int max_nb, double dt,
                                              hipEvent_t* ev) {
   const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
   const int bd = ctx->solve_block, wpb = bd / 64;
   const int ngroups = nbig + (n - nbig + wpb - 1) / wpb;
   hipStream_t st = ctx->stream;
   if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
-  k_bp<R><<<n, 256, 0, st>>>(dw, d_list);
+  {
+    // naive broadphase body data in LDS when it fits beside the static LDS (4 KiB kept free)
+    const size_t per_body = BpLds<R>::bytes(1);
+    const int lds_nb = std::min(max_nb, (int)((ctx->max_lds - 4096) / per_body));
+    k_bp<R><<<n, 256, BpLds<R>::bytes(lds_nb), st>>>(dw, d_list, lds_nb);
+  }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
   {
     const int qcap = (int)ctx->d_q_cap;
@@ -721,7 +742,8 @@ template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, ui
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int s = 0; s < nsteps; s++) {
-    r = launch_step<R>(ctx, ctx->d_list, (int)n, nbig, dt, ctx->profile ? &ctx->sev[s * (K_NSTAGE + 1)] : nullptr);
+    r = launch_step<R>(ctx, ctx->d_list, (int)n, nbig, ctx->worlds[list[0]].nb, dt,
+                       ctx->profile ? &ctx->sev[s * (K_NSTAGE + 1)] : nullptr);
     if (r != CHP_OK) return r;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -765,7 +787,7 @@ template <typename R> static int step_plain_t(chp_ctx* ctx, uint32_t id, double 
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int s = 0; s < nsteps; s++) {
-    r = launch_step<R>(ctx, ctx->d_list, 1, ctx->worlds[id].nb > SMALL_NB ? 1 : 0, dt, nullptr);
+    r = launch_step<R>(ctx, ctx->d_list, 1, ctx->worlds[id].nb > SMALL_NB ? 1 : 0, ctx->worlds[id].nb, dt, nullptr);
     if (r != CHP_OK) return r;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the slot-by-slot drop-in hands back each step
   }
@@ -1163,6 +1185,7 @@ int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   c->budget = hbm_budget ? hbm_budget : (uint64_t)(0.9 * (double)fr);
+  c->max_lds = (size_t)prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CHP_E_HIP; }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return CHP_E_HIP; }
   if (const char* e = getenv("CHP_SOLVE_BLOCK")) c->solve_block = atoi(e);

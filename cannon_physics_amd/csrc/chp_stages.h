@@ -144,14 +144,60 @@ template <typename R> __device__ __forceinline__ bool bp_sphere(const DevWorld<R
 __device__ __forceinline__ float axis_of(f3 v, int ax) { return ax == 0 ? v.x : (ax == 1 ? v.y : v.z); }
 
 // NaiveBroadphase.collisionPairs (naive_broadphase.dart:14-33): pairs (i, j), j < i, in i-major
-// order; counted, scanned, then written in order
-template <typename R> __device__ bool st_bp_naive(const DevWorld<R>& W, int* sh) {
-  int n = W.nb;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+// order.  One wave per row i (rows dealt round-robin to the waves): the lanes test 64
+// consecutive j at a time, a ballot gives each passing lane its rank, so pairs come out in j
+// order.  Pass 1 counts per row, a scan gives the row offsets, pass 2 writes.  The body data
+// the tests read (position, bounding radius, filter group / mask, static flag) is staged in LDS
+// when the world fits (BpLds), else read from the slab.
+template <typename R> struct BpLds {
+  float* pos;  // 3 per body
+  R* brad;
+  int* gm;     // group, mask per body
+  int* stat;   // 1: static
+  static __host__ __device__ size_t bytes(int n) { return (size_t)n * (3 * sizeof(float) + sizeof(R) + 3 * sizeof(int)); }
+  __device__ void carve(void* base, int cap) {
+    brad = (R*)base;
+    pos = (float*)(brad + cap);
+    gm = (int*)(pos + 3 * cap);
+    stat = gm + 2 * cap;
+  }
+};
+template <typename R, bool LDS>
+__device__ __forceinline__ bool bp_test(const DevWorld<R>& W, const BpLds<R>& L, int a, int b) {
+  if constexpr (LDS) {
+    if ((L.gm[2 * a] & L.gm[2 * b + 1]) == 0 || (L.gm[2 * b] & L.gm[2 * a + 1]) == 0) return false;
+    if (L.stat[a] && L.stat[b]) return false;
+    f3 pa = mk3(L.pos[3 * a], L.pos[3 * a + 1], L.pos[3 * a + 2]), pb = mk3(L.pos[3 * b], L.pos[3 * b + 1], L.pos[3 * b + 2]);
+    f3 r = sub3<double>(pb, pa);
+    double s = (double)L.brad[a] + (double)L.brad[b];
+    return len2<double>(r) < s * s;
+  } else {
+    return bp_need(W, a, b) && bp_sphere(W, a, b);
+  }
+}
+template <typename R, bool LDS>
+__device__ __forceinline__ unsigned long long bp_row_chunk(const DevWorld<R>& W, const BpLds<R>& L, int i, int j0) {
+  int j = j0 + (int)(threadIdx.x & 63);
+  bool hit = j < i && bp_test<R, LDS>(W, L, i, j);
+  return __ballot(hit);
+}
+template <typename R, bool LDS> __device__ bool st_bp_naive(const DevWorld<R>& W, int* sh, const BpLds<R>& L) {
+  const int n = W.nb, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  if constexpr (LDS) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      f3 p = F3<R>(W.pos, i);
+      L.pos[3 * i] = p.x; L.pos[3 * i + 1] = p.y; L.pos[3 * i + 2] = p.z;
+      L.brad[i] = W.brad[i];
+      L.gm[2 * i] = W.group[i];
+      L.gm[2 * i + 1] = W.mask[i];
+      L.stat[i] = W.type[i] == CHP_BODY_STATIC;
+    }
+    __syncthreads();
+  }
+  for (int i = wave; i < n; i += nwave) {
     int c = 0;
-    for (int j = 0; j < i; j++)
-      if (bp_need(W, i, j) && bp_sphere(W, i, j)) c++;
-    W.cnt[i] = c;
+    for (int j0 = 0; j0 < i; j0 += 64) c += __popcll(bp_row_chunk<R, LDS>(W, L, i, j0));
+    if (lane == 0) W.cnt[i] = c;
   }
   __syncthreads();
   int P = array_exscan(W.cnt, n, sh);
@@ -159,10 +205,17 @@ template <typename R> __device__ bool st_bp_naive(const DevWorld<R>& W, int* sh)
     if (threadIdx.x == 0) { W.counters[C_PAIRS] = P; ws_fail(W.counters, WS_PAIRS); }
     return false;
   }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = wave; i < n; i += nwave) {
     int o = W.cnt[i];
-    for (int j = 0; j < i; j++)
-      if (bp_need(W, i, j) && bp_sphere(W, i, j)) { W.pair_i[o] = i; W.pair_j[o] = j; o++; }
+    for (int j0 = 0; j0 < i; j0 += 64) {
+      unsigned long long m = bp_row_chunk<R, LDS>(W, L, i, j0);
+      if ((m >> lane) & 1ull) {
+        int k = o + __popcll(m & ((1ull << lane) - 1ull));
+        W.pair_i[k] = i;
+        W.pair_j[k] = j0 + lane;
+      }
+      o += __popcll(m);
+    }
   }
   if (threadIdx.x == 0) W.counters[C_PAIRS] = P;
   __syncthreads();
