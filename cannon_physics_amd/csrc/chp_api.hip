@@ -478,7 +478,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   SCR(contacts, ContactRec<R>, contact_cap);
   SCR(rows, RowRec<R>, row_cap);
   SCR(level, int, row_cap); SCR(sched, int, row_cap); SCR(lvl_start, int, row_cap + 2);
-  SCR(row_ab, int, 2 * row_cap); SCR(rpos, int, row_cap); SCR(absdl, R, row_cap + 1); SCR(glam, R, row_cap + 1);
+  SCR(row_ab, int, 2 * row_cap); SCR(rpos, int, row_cap + 1); SCR(absdl, R, row_cap + 1); SCR(glam, R, row_cap + 1);
   fix.emplace_back((void**)(void*)&D.gsr, L.add<R>((size_t)4 * GS_GROUPS * row_cap));
   SCR(vw, float, 8 * nb);
   SCR(last_level, int, std::max(nb, row_cap + 1));

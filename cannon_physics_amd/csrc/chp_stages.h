@@ -914,34 +914,66 @@ template <typename R> __device__ __forceinline__ void gs_store(typename GsWord<R
   }
 }
 
-// Levels.  One wave walks the rows in solver order, 64 at a time: a row starts one level above
-// the last level of each of its dynamic bodies (last[], from the earlier chunks); inside the
-// chunk lane j's level is final once lanes 0..j-1 are done, so a uniform pass over j hands it
-// to the later lanes sharing a body.  The rows are then counting-sorted by level and gathered
+// Levels.  Consecutive rows on the same two bodies (a contact's two friction rows, the rows of
+// one box-box contact patch, a joint's rows) form a run; inside a run each row depends on the
+// one before it and on nothing later outside the run, so the run's rows take consecutive levels
+// from the run's start level.  One wave walks the runs in solver order, 64 at a time: a run
+// starts one level above the last level of each of its dynamic bodies (last[], from the earlier
+// chunks); inside the chunk lane j's start is final once lanes 0..j-1 are done, so a uniform
+// pass over j hands its end level to the later lanes sharing a body.  Rows without a dynamic body
+// depend on nothing and take level 0.  The rows are then counting-sorted by level and gathered
 // into the GS records in level order.
 //   last: nb ints (LDS when LDS, else the world's global scratch, read with atomics so that
 //         each chunk sees the previous chunk's values)
 //   hist: hist_cap ints of LDS for the level histogram (global scratch when L + 1 > hist_cap
 //         or not LDS)
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <typename R, typename G, bool LDS>
 __device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int* sh, int* last, int* hist, int hist_cap) {
-  const int E = W.counters[C_ROWS], nb = W.nb;
+  const int E = uni(W.counters[C_ROWS]), nb = W.nb;
   const int tid = g.tid, bd = g.n;
+  const int2* ab = (const int2*)W.row_ab;
+  int* runid = W.level;     // run of each row, then its level
+  int* rstart = W.rpos;     // first row of each run (+ E at the end), until the sort below
+  int* rlev = W.sched;      // start level of each run, until the sort below
   for (int b = tid; b < nb; b += bd) last[b] = -1;
+  // runs: a row starts one when its body pair differs from the previous row's
+  for (int r = tid; r < E; r += bd) {
+    int2 x = ab[r];
+    runid[r] = (r == 0 || x.x != ab[r - 1].x || x.y != ab[r - 1].y) ? 1 : 0;
+  }
+  g.sync();
+  const int NR = uni(g.exscan(runid, E, sh));
+  for (int r = tid; r < E; r += bd) {
+    int2 x = ab[r];
+    bool first = r == 0 || x.x != ab[r - 1].x || x.y != ab[r - 1].y;
+    int id = runid[r] + (first ? 1 : 0) - 1;
+    runid[r] = id;
+    if (first) rstart[id] = r;
+  }
+  if (tid == 0) rstart[NR] = E;
   g.sync();
   int lmax = 0;
   if (tid < 64) {
     const int lane = tid;
-    int2 ab = make_int2(0, 0);
-    if (lane < E) ab = ((const int2*)W.row_ab)[lane];
-    for (int c0 = 0; c0 < E; c0 += 64) {
-      int r = c0 + lane;
-      bool act = r < E;
-      int2 nab = make_int2(0, 0);   // next chunk, loaded ahead
-      if (r + 64 < E) nab = ((const int2*)W.row_ab)[r + 64];
-      const bool da = act && (ab.x & GS_DYN), db = act && (ab.y & GS_DYN);
-      const int a = ab.x & GS_IDX, b = ab.y & GS_IDX;
+    for (int c0 = 0; c0 < NR; c0 += 64) {
+      const int q = c0 + lane;
+      const bool act = q < NR;
+      int a = 0, b = 0, len = 0;
+      bool da = false, db = false;
+      if (act) {
+        int s0 = rstart[q];
+        int2 x = ab[s0];
+        len = rstart[q + 1] - s0;
+        da = x.x & GS_DYN;
+        db = x.y & GS_DYN;
+        a = x.x & GS_IDX;
+        b = x.y & GS_IDX;
+      }
+      const bool dyn = da || db;
       const int ka = da ? a : -1 - 2 * lane, kb = db ? b : -2 - 2 * lane;   // non-dynamic keys never match
+      const int span = dyn ? len : 1;   // levels a run occupies
       int lev = 0;
       if constexpr (LDS) {
         if (da) lev = max(lev, last[a] + 1);
@@ -950,20 +982,28 @@ __device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int*
         if (da) lev = max(lev, atomicAdd(&last[a], 0) + 1);
         if (db) lev = max(lev, atomicAdd(&last[b], 0) + 1);
       }
-      const int n = min(64, E - c0);
+      const int n = uni(min(64, NR - c0));
       for (int j = 0; j < n - 1; j++) {
         int kaj = __builtin_amdgcn_readlane(ka, j), kbj = __builtin_amdgcn_readlane(kb, j);
-        int lj = __builtin_amdgcn_readlane(lev, j);
-        if (lane > j && (ka == kaj || ka == kbj || kb == kaj || kb == kbj)) lev = max(lev, lj + 1);
+        int endj = __builtin_amdgcn_readlane(lev + span - 1, j);
+        bool hit = lane > j && (ka == kaj || ka == kbj || kb == kaj || kb == kbj);
+        lev = hit ? max(lev, endj + 1) : lev;
       }
-      // the chain through a body has increasing levels: the chunk's last row on it wins
-      if (da) atomicMax(&last[a], lev);
-      if (db) atomicMax(&last[b], lev);
-      if (act) { W.level[r] = lev; lmax = max(lmax, lev + 1); }
-      ab = nab;
+      // the chain through a body has increasing levels: the chunk's last run on it wins
+      if (da) atomicMax(&last[a], lev + span - 1);
+      if (db) atomicMax(&last[b], lev + span - 1);
+      if (act) {
+        rlev[q] = dyn ? lev : -1;
+        lmax = max(lmax, lev + span);
+      }
     }
   }
-  const int L = g.imax(lmax, sh);
+  const int L = uni(g.imax(lmax, sh));
+  for (int r = tid; r < E; r += bd) {
+    int id = runid[r], l0 = rlev[id];
+    runid[r] = l0 < 0 ? 0 : l0 + (r - rstart[id]);
+  }
+  g.sync();
   // counting sort by level; the order inside a level is free (its rows are independent)
   auto sort = [&](int* cnt) {
     for (int l = tid; l <= L; l += bd) cnt[l] = 0;
@@ -987,8 +1027,8 @@ __device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int*
   for (int k = tid; k < E; k += bd) {
     int r = W.sched[k];
     RowRec<R> o = W.rows[r];
-    int2 ab = ((const int2*)W.row_ab)[r];
-    gs_store<R>(gsr, k, o, (unsigned)ab.x, (unsigned)ab.y);
+    int2 x = ab[r];
+    gs_store<R>(gsr, k, o, (unsigned)x.x, (unsigned)x.y);
     W.glam[k] = (R)0;
   }
   if (tid == 0) W.counters[C_LEVELS] = L;
@@ -1055,8 +1095,6 @@ __device__ __forceinline__ R gs_row(const GsRow<R>& cr, R clam, bool v, int ks, 
   }
   return lnew;
 }
-
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // vw: 8 floats per body (vlambda xyz, pad, wlambda xyz, pad).  LDS: vw and ls_lds (L + 1 ints)
 // are LDS arrays; else vw is the world's global scratch and the level starts are read from
