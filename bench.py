@@ -216,7 +216,8 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--worlds", type=int, default=None)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--solve-block", type=int, default=256)
+    ap.add_argument("--solve-block", type=int, default=0,
+                    help="GS workgroup size (64, 128, 256); 0: the autotuner picks it during warm-up")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -243,12 +244,20 @@ def main():
 
     prec = A.PRECISION_F64 if args.precision == "f64" else A.PRECISION_F32
     ctx = Context(device=0, precision=prec)
-    ctx.set_tuning(solve_block=args.solve_block)
+    if args.solve_block:
+        ctx.set_tuning(solve_block=args.solve_block)
     wids = [ctx.create_world(r) for r in recs]
     log("uploaded: %.2f GB on the device" % (ctx.bytes_in_use() / 1e9))
 
     if args.warmup:
-        ctx.step(wids, dt, args.warmup)
+        # the autotuner times the level / GS workgroup sizes on the last warm-up step, once
+        # contacts have formed (the choice does not change results)
+        if args.warmup > 1:
+            ctx.step(wids, dt, args.warmup - 1)
+        if not args.solve_block:
+            ctx.autotune()
+        ctx.step(wids, dt, 1)
+    log("launch shape: " + json.dumps(ctx.tuning()))
     before = stats_totals(ctx, wids)
     ctx.set_profiling(True)
     ctx.sync()
@@ -269,6 +278,12 @@ def main():
         % (K, wall, dev_ms, P, Cn, E))
     log("stage ms over the timed region: " + json.dumps({k: round(v, 2) for k, v in stages.items()}))
 
+    # the same number of steps again without per-stage events: chp_step replays a captured
+    # graph of the step
+    t0 = time.perf_counter()
+    ctx.step(wids, dt, K)
+    graph_wall = time.perf_counter() - t0
+    log("graph replay: %d steps in %.3fs" % (K, graph_wall))
     dom = max(stages, key=stages.get)
     gs_ms = stages["k_gs"] / K
     achieved_gs = B_ROW_GS * E / (gs_ms / 1e3) / 1e9
@@ -308,12 +323,14 @@ def main():
         "steps": K,
         "warmup": args.warmup,
         "ms_per_step": wall / K * 1e3,
+        "graph_replay_ms_per_step": graph_wall / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.precision,
         "data": "synthetic",
         "config": {"workload": desc, "worlds": len(recs), "dynamic_bodies": n_dyn, "dt": dt,
+                   "launch_shape": ctx.tuning(),
                    "pairs_per_step": P, "contacts_per_step": Cn, "rows_per_step": E},
         "roofline": {"bound": "hbm", "kernel": "k_gs", "achieved": achieved_gs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved_gs / HBM_PEAK_GBS,

@@ -48,6 +48,9 @@ def load():
         "chp_sync": (i32, [ctx_p]),
         "chp_last_step_ms": (i32, [ctx_p, P(dbl)]),
         "chp_ctx_set_tuning": (i32, [ctx_p, C.c_int, C.c_int]),
+        "chp_ctx_autotune": (i32, [ctx_p]),
+        "chp_ctx_set_graphs": (i32, [ctx_p, C.c_int]),
+        "chp_ctx_get_tuning": (i32, [ctx_p, P(C.c_int), P(C.c_int)]),
         "chp_ctx_set_profiling": (i32, [ctx_p, C.c_int]),
         "chp_stage_times": (i32, [ctx_p, P(dbl), P(C.c_char_p), u32, P(u32)]),
         "chp_world_stats": (i32, [ctx_p, u32, P(A.StepStats)]),

@@ -359,7 +359,21 @@ struct chp_ctx {
   bool profile = false;                      // per-stage events in chp_step
   std::vector<hipEvent_t> sev;               // 7 events per step
   std::vector<double> stage_ms;              // accumulated per stage over the last chp_step
-  int solve_block = 256;   // workgroup of k_levels / k_gs: one large world or blockDim/64 small ones
+  // workgroup sizes of k_levels / k_gs: one large world, or blockDim/64 small ones, per workgroup
+  int levels_block = 256, solve_block = 128;
+  bool tune_pending = false;                 // time the candidates in the next waited step
+  // hipGraph of one step of the last batched configuration, replayed once per step
+  struct StepGraph {
+    bool valid = false;
+    std::vector<int> list;
+    int nbig = 0, max_nb = 0, bl = 0, bg = 0;
+    double dt = 0;
+    const void *dq = nullptr, *dqcls = nullptr, *dlist = nullptr, *dworlds = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } sg;
+  bool use_graphs = true;
+  std::vector<hipEvent_t> tev;               // 3 events per candidate
   int2* d_q = nullptr;           // narrowphase queue of a step (world, pair), sorted by resolver class
   size_t d_q_cap = 0;
   int* d_qcls = nullptr;         // per-world class counts / offsets, then the class bounds
@@ -677,11 +691,15 @@ static const char* kStageNames[K_NSTAGE] = {"k_bp", "k_narrow", "k_rows", "k_lev
 
 // launches the stage kernels of one step for the worlds in d_list[0..n); ev (optional) gets
 // K_NSTAGE + 1 events around them
+static const int kTuneBlocks[3] = {64, 128, 256};
+
+// the stage kernels of one step.  tune: the level and GS stages run once per candidate
+// workgroup size, timed by ctx->tev (they rebuild the same schedule from the same rows, so every
+// run leaves the same result)
 template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, int n, int nbig, int max_nb, double dt,
-                                             hipEvent_t* ev) {
+                                             hipEvent_t* ev, bool tune = false) {
   const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
-  const int bd = ctx->solve_block, wpb = bd / 64;
-  const int ngroups = nbig + (n - nbig + wpb - 1) / wpb;
+  auto groups = [&](int bd) { return nbig + (n - nbig + bd / 64 - 1) / (bd / 64); };
   hipStream_t st = ctx->stream;
   if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
   {
@@ -705,13 +723,66 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
   k_rows<R><<<n, 128, 0, st>>>(dw, d_list, dt);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
-  k_levels<R><<<ngroups, bd, lds_levels(bd), st>>>(dw, d_list, nbig, n);
-  if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
-  k_gs<R><<<ngroups, bd, lds_gs(bd), st>>>(dw, d_list, nbig, n);
-  if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
+  if (tune) {
+    for (int c = 0; c < 3; c++) {
+      const int b = kTuneBlocks[c];
+      HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c], st));
+      k_levels<R><<<groups(b), b, lds_levels(b), st>>>(dw, d_list, nbig, n);
+      HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 1], st));
+      k_gs<R><<<groups(b), b, lds_gs(b), st>>>(dw, d_list, nbig, n);
+      HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 2], st));
+    }
+    if (ev) {
+      HIPCHK(ctx, hipEventRecord(ev[4], st));
+      HIPCHK(ctx, hipEventRecord(ev[5], st));
+    }
+  } else {
+    const int bl = ctx->levels_block, bg = ctx->solve_block;
+    k_levels<R><<<groups(bl), bl, lds_levels(bl), st>>>(dw, d_list, nbig, n);
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
+    k_gs<R><<<groups(bg), bg, lds_gs(bg), st>>>(dw, d_list, nbig, n);
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
+  }
   k_post<R><<<n, 256, 0, st>>>(dw, d_list, dt);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
   HIPCHK(ctx, hipGetLastError());
+  return CHP_OK;
+}
+
+static void drop_graph(chp_ctx* ctx) {
+  if (ctx->sg.exec) (void)hipGraphExecDestroy(ctx->sg.exec);
+  if (ctx->sg.graph) (void)hipGraphDestroy(ctx->sg.graph);
+  ctx->sg = chp_ctx::StepGraph();
+}
+
+// the captured step for this configuration (captured anew when anything baked into its kernel
+// arguments changed)
+template <typename R>
+static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int max_nb, double dt, hipGraphExec_t* out) {
+  chp_ctx::StepGraph& g = ctx->sg;
+  if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block &&
+        g.bg == ctx->solve_block && g.dq == ctx->d_q && g.dqcls == ctx->d_qcls && g.dlist == ctx->d_list &&
+        g.dworlds == ctx->d_worlds)) {
+    drop_graph(ctx);
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    int r = launch_step<R>(ctx, ctx->d_list, (int)list.size(), nbig, max_nb, dt, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (r != CHP_OK) { if (graph) (void)hipGraphDestroy(graph); return r; }
+    if (e != hipSuccess) return fail(ctx, CHP_E_HIP, std::string("step capture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(ctx, CHP_E_HIP, std::string("step graph: ") + hipGetErrorString(e));
+    }
+    g.valid = true; g.list = list; g.nbig = nbig; g.max_nb = max_nb; g.dt = dt;
+    g.bl = ctx->levels_block; g.bg = ctx->solve_block;
+    g.dq = ctx->d_q; g.dqcls = ctx->d_qcls; g.dlist = ctx->d_list; g.dworlds = ctx->d_worlds;
+    g.graph = graph;
+    g.exec = exec;
+  }
+  *out = g.exec;
   return CHP_OK;
 }
 
@@ -740,10 +811,27 @@ template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, ui
     HIPCHK(ctx, hipEventCreate(&e));
     ctx->sev.push_back(e);
   }
+  const bool tune = ctx->tune_pending && wait && nsteps > 0;
+  while (tune && ctx->tev.size() < 9) {
+    hipEvent_t e;
+    HIPCHK(ctx, hipEventCreate(&e));
+    ctx->tev.push_back(e);
+  }
+  const int max_nb = ctx->worlds[list[0]].nb;
+  // steps after the first of a call replay a captured graph of the step (no per-step events)
+  hipGraphExec_t exec = nullptr;
+  if (ctx->use_graphs && !ctx->profile && nsteps > 1) {
+    r = step_graph<R>(ctx, list, nbig, max_nb, dt, &exec);
+    if (r != CHP_OK) return r;
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int s = 0; s < nsteps; s++) {
-    r = launch_step<R>(ctx, ctx->d_list, (int)n, nbig, ctx->worlds[list[0]].nb, dt,
-                       ctx->profile ? &ctx->sev[s * (K_NSTAGE + 1)] : nullptr);
+    if (exec && !(tune && s == 0)) {
+      HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
+      continue;
+    }
+    r = launch_step<R>(ctx, ctx->d_list, (int)n, nbig, max_nb, dt,
+                       ctx->profile ? &ctx->sev[s * (K_NSTAGE + 1)] : nullptr, tune && s == 0);
     if (r != CHP_OK) return r;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -751,6 +839,17 @@ template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, ui
   ctx->stage_ms.assign(K_NSTAGE, 0.0);
   if (!wait && !ctx->profile) return CHP_OK;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (tune) {  // keep the fastest workgroup size of each stage
+    float bestl = 1e30f, bestg = 1e30f;
+    for (int c = 0; c < 3; c++) {
+      float tl = 0, tg = 0;
+      HIPCHK(ctx, hipEventElapsedTime(&tl, ctx->tev[3 * c], ctx->tev[3 * c + 1]));
+      HIPCHK(ctx, hipEventElapsedTime(&tg, ctx->tev[3 * c + 1], ctx->tev[3 * c + 2]));
+      if (tl < bestl) { bestl = tl; ctx->levels_block = kTuneBlocks[c]; }
+      if (tg < bestg) { bestg = tg; ctx->solve_block = kTuneBlocks[c]; }
+    }
+    ctx->tune_pending = false;
+  }
   if (ctx->profile)
     for (int s = 0; s < nsteps; s++)
       for (int k = 0; k < K_NSTAGE; k++) {
@@ -1012,11 +1111,11 @@ static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t 
   int one = (int)id;
   HIPCHK(ctx, hipMemcpy(ctx->d_list, &one, sizeof(int), hipMemcpyHostToDevice));
   const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
-  const int bd = ctx->solve_block, nbig = S.nb > SMALL_NB ? 1 : 0;
+  const int bl = ctx->levels_block, bg = ctx->solve_block, nbig = S.nb > SMALL_NB ? 1 : 0;
   hipStream_t st = ctx->stream;
   k_solve_rows<R><<<1, 256, 0, st>>>(dw, (int)id, ctx->d_eq, (int)n, dt);
-  k_levels<R><<<1, bd, lds_levels(bd), st>>>(dw, ctx->d_list, nbig, 1);
-  k_gs<R><<<1, bd, lds_gs(bd), st>>>(dw, ctx->d_list, nbig, 1);
+  k_levels<R><<<1, bl, lds_levels(bl), st>>>(dw, ctx->d_list, nbig, 1);
+  k_gs<R><<<1, bg, lds_gs(bg), st>>>(dw, ctx->d_list, nbig, 1);
   k_solve_finish<R><<<1, 256, 0, st>>>(dw, (int)id);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(st));
@@ -1188,8 +1287,7 @@ int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out
   c->max_lds = (size_t)prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CHP_E_HIP; }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return CHP_E_HIP; }
-  if (const char* e = getenv("CHP_SOLVE_BLOCK")) c->solve_block = atoi(e);
-  if (c->solve_block != 64 && c->solve_block != 128 && c->solve_block != 256) c->solve_block = 256;
+
   *out = c;
   return CHP_OK;
 }
@@ -1198,6 +1296,7 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   if (!ctx) return CHP_E_INVALID;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
   ctx->arena.destroy();
   if (ctx->d_worlds) (void)hipFree(ctx->d_worlds);
   if (ctx->d_list) (void)hipFree(ctx->d_list);
@@ -1207,6 +1306,7 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   if (ctx->d_q) (void)hipFree(ctx->d_q);
   if (ctx->d_qcls) (void)hipFree(ctx->d_qcls);
   for (hipEvent_t e : ctx->sev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipStreamDestroy(ctx->stream);
@@ -1493,17 +1593,34 @@ int chp_stage_times(chp_ctx* ctx, double* ms, const char** names, uint32_t cap, 
   return CHP_OK;
 }
 
-int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int lds_cap_bodies) {
+int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int levels_block) {
   if (!ctx) return CHP_E_INVALID;
-  if (solve_block != 0) {
-    if (solve_block != 64 && solve_block != 128 && solve_block != 256)
-      return fail(ctx, CHP_E_INVALID, "solve_block must be 64, 128 or 256");
-    ctx->solve_block = solve_block;
-  }
-  if (lds_cap_bodies > 0) {
-    if (lds_cap_bodies > 2560) return fail(ctx, CHP_E_INVALID, "lds_cap_bodies above 2560 exceeds 64 KiB of LDS");
-    ctx->lds_cap_bodies = lds_cap_bodies;
-  }
+  auto ok = [](int b) { return b == 0 || b == 64 || b == 128 || b == 256; };
+  if (!ok(solve_block) || !ok(levels_block)) return fail(ctx, CHP_E_INVALID, "workgroup sizes must be 64, 128 or 256");
+  if (solve_block) ctx->solve_block = solve_block;
+  if (levels_block) ctx->levels_block = levels_block;
+  return CHP_OK;
+}
+
+int chp_ctx_set_graphs(chp_ctx* ctx, int on) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  ctx->use_graphs = on != 0;
+  if (!ctx->use_graphs) drop_graph(ctx);
+  return CHP_OK;
+}
+
+int chp_ctx_autotune(chp_ctx* ctx) {
+  if (!ctx) return CHP_E_INVALID;
+  ctx->tune_pending = true;
+  return CHP_OK;
+}
+
+int chp_ctx_get_tuning(chp_ctx* ctx, int* solve_block, int* levels_block) {
+  if (!ctx || !solve_block || !levels_block) return CHP_E_INVALID;
+  *solve_block = ctx->solve_block;
+  *levels_block = ctx->levels_block;
   return CHP_OK;
 }
 

@@ -99,8 +99,20 @@ class Context:
         self.check(self.lib.chp_stage_times(self.ptr, ms, names, 16, C.byref(n)))
         return {names[k].decode(): float(ms[k]) for k in range(n.value)}
 
-    def set_tuning(self, solve_block=0, lds_cap_bodies=0):
-        self.check(self.lib.chp_ctx_set_tuning(self.ptr, solve_block, lds_cap_bodies))
+    def set_tuning(self, solve_block=0, levels_block=0):
+        self.check(self.lib.chp_ctx_set_tuning(self.ptr, solve_block, levels_block))
+
+    def set_graphs(self, on=True):
+        self.check(self.lib.chp_ctx_set_graphs(self.ptr, int(on)))
+
+    def autotune(self):
+        """the next waited step times the level / GS workgroup sizes and keeps the fastest"""
+        self.check(self.lib.chp_ctx_autotune(self.ptr))
+
+    def tuning(self):
+        a, b = C.c_int(), C.c_int()
+        self.check(self.lib.chp_ctx_get_tuning(self.ptr, C.byref(a), C.byref(b)))
+        return {"solve_block": a.value, "levels_block": b.value}
 
     def step_plain(self, wid, dt, nsteps=1):
         self.check(self.lib.chp_step_plain(self.ptr, wid, dt, nsteps))

@@ -254,10 +254,18 @@ int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, doubl
 int chp_sync(chp_ctx* ctx);
 /* device time (HIP events on the context stream) of the last chp_step / chp_step_plain call */
 int chp_last_step_ms(chp_ctx* ctx, double* ms);
-/* launch shape of the level and GS stages: workgroup size (64, 128 or 256; 0 keeps).  A large
- * world takes a workgroup, small worlds (<= 128 bodies) take one wave each.  lds_cap_bodies is
- * accepted for compatibility and ignored. */
-int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int lds_cap_bodies);
+/* launch shape of the GS and level stages: workgroup sizes (64, 128 or 256; 0 keeps).  A large
+ * world takes a workgroup, small worlds (<= 128 bodies) take one wave each.  Results do not
+ * depend on the choice (the GS schedule is exact whatever the grouping). */
+int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int levels_block);
+/* Autotuner: the next waited chp_step runs the level and GS stages of its first step once per
+ * workgroup size (64, 128, 256), times them with HIP events and keeps the fastest of each
+ * (every run leaves the same result, so the step is the step chp_step would take). */
+int chp_ctx_autotune(chp_ctx* ctx);
+int chp_ctx_get_tuning(chp_ctx* ctx, int* solve_block, int* levels_block);
+/* chp_step with nsteps > 1 captures one step's kernels in a hipGraph and replays it for each
+ * step (on by default; off: every step is launched kernel by kernel).  Same results either way. */
+int chp_ctx_set_graphs(chp_ctx* ctx, int on);
 /* per-stage HIP events in chp_step (costs one event per stage kernel) */
 int chp_ctx_set_profiling(chp_ctx* ctx, int on);
 /* device milliseconds of each stage kernel summed over the last profiled chp_step; names get

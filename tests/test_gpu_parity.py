@@ -211,3 +211,47 @@ def test_full_batch_of_4096_worlds_runs_and_samples_match_oracle():
     finally:
         for w in ids:
             c.destroy_world(w)
+
+
+def test_workgroup_sizes_and_autotune_leave_results_unchanged():
+    """The GS schedule is exact whatever the grouping: every level / GS workgroup size, and the
+    autotuner's timed reruns, give bit-identical worlds."""
+    c = ctx("f32")
+    recs = [scenes.c3_world(i).recipe() for i in (300, 301, 302, 303)] + [scenes.c4_chain_world(9).recipe()]
+    results = []
+    for tune in [(64, 256), (256, 64), (128, 128), "auto"]:
+        ids = [c.create_world(r) for r in recs]
+        if tune == "auto":
+            c.autotune()
+        else:
+            c.set_tuning(solve_block=tune[0], levels_block=tune[1])
+        c.step(ids, DT, 15)
+        results.append([state_to_numpy(*c.download(w)) for w in ids])
+        for w in ids:
+            c.destroy_world(w)
+    assert c.tuning()["solve_block"] in (64, 128, 256)
+    for other in results[1:]:
+        for a, b in zip(results[0], other):
+            assert all(same_state(a, b).values())
+
+
+def test_graph_replays_equal_eager_steps():
+    """chp_step replays a captured step graph: several replays, with eager steps and downloads in
+    between, equal the kernel-by-kernel path bit for bit after every call."""
+    recs = [scenes.c3_world(i).recipe() for i in (310, 311, 312)] + [scenes.c4_chain_world(19).recipe()]
+    g, e = Context(device=0), Context(device=0)
+    e.set_graphs(False)
+    try:
+        gi = [g.create_world(r) for r in recs]
+        ei = [e.create_world(r) for r in recs]
+        for call in range(4):
+            g.step(gi, DT, 6)
+            e.step(ei, DT, 6)
+            if call == 1:   # eager work on the graph context between replays
+                g.step(gi, DT, 1)
+                e.step(ei, DT, 1)
+            for a, b in zip(gi, ei):
+                assert all(same_state(state_to_numpy(*g.download(a)), state_to_numpy(*e.download(b))).values()), call
+    finally:
+        g.close()
+        e.close()
