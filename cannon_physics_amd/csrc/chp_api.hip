@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <deque>
 #include <map>
@@ -38,6 +39,7 @@ __global__ __launch_bounds__(256) void k_bp(const DevWorld<R>* __restrict__ worl
   extern __shared__ double lds_d[];
   CHP_WORLD_PROLOGUE
   if (threadIdx.x == 0) W.counters[C_STAGING] = 0;
+  if (W.nb >= BP_BIG) return;  // the grid-wide kernels below take large worlds
   st_prestep(W);
   __syncthreads();
   bool ok;
@@ -53,6 +55,64 @@ __global__ __launch_bounds__(256) void k_bp(const DevWorld<R>* __restrict__ worl
     }
   }
   if (ok) st_filter(W);
+}
+
+// large worlds (at least BP_BIG bodies): prestep, SAP sort and pair passes over the grid.
+// spans: (list index, first row) per BP_SPAN rows; tiles: (list index, first position) per
+// SAP_TILE keys of the SAP worlds; bigw: list indices of the large worlds.
+#define CHP_SPAN_PROLOGUE(tab)                                      \
+  const int2 sp = tab[blockIdx.x];                                  \
+  const DevWorld<R>& W = worlds[list[sp.x]];                        \
+  if (W.counters[C_STATUS] != 0) return;
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_big_prestep(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                    const int2* __restrict__ spans) {
+  CHP_SPAN_PROLOGUE(spans)
+  const int i = sp.y + threadIdx.x;
+  if (i < W.nb) prestep_body(W, i);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_sap_tile(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                  const int2* __restrict__ tiles, int apply) {
+  __shared__ unsigned long long lk[SAP_TILE];
+  __shared__ int lord[SAP_TILE];
+  CHP_SPAN_PROLOGUE(tiles)
+  sap_tile(W, sp.y, lk, lord, apply != 0);
+}
+template <typename R>
+__global__ __launch_bounds__(64) void k_sap_merge(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                  const int2* __restrict__ spans, int w, int src, int apply) {
+  CHP_SPAN_PROLOGUE(spans)
+  const int g = sp.y + threadIdx.x;
+  if (W.broadphase == CHP_BROADPHASE_SAP && g < W.nb) sap_merge(W, g, w, src, apply != 0);
+}
+// pass 0 counts the pairs of each row into cnt; pass 1 writes them from the scanned offsets
+template <typename R>
+__global__ __launch_bounds__(256) void k_big_rows(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                  const int2* __restrict__ spans, int emit) {
+  CHP_SPAN_PROLOGUE(spans)
+  const int end = min(sp.y + BP_SPAN, W.nb);
+  for (int k = sp.y + (int)(threadIdx.x >> 6); k < end; k += 4) {
+    if (emit) {
+      big_row(W, k, true, W.cnt[k]);
+    } else {
+      int c = big_row(W, k, false, 0);
+      if ((threadIdx.x & 63) == 0) W.cnt[k] = c;
+    }
+  }
+}
+template <typename R>
+__global__ __launch_bounds__(1024) void k_big_scan(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                   const int* __restrict__ bigw) {
+  __shared__ int sh[1024];
+  const DevWorld<R>& W = worlds[list[bigw[blockIdx.x]]];
+  if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
+  const int P = array_exscan(W.cnt, W.nb, sh);
+  if (threadIdx.x == 0) {
+    W.counters[C_PAIRS] = P;
+    if (P > W.pair_cap) ws_fail(W.counters, WS_PAIRS);
+  }
 }
 
 // narrowphase: per-class pair counts per world, class-major offsets, placement in the queue,
@@ -323,6 +383,7 @@ class Arena {
 struct WorldSlot {
   bool live = false;
   int nb = 0;
+  int broadphase = CHP_BROADPHASE_NAIVE;
   void* mem = nullptr;
   size_t bytes = 0;
   double last_dt = -1;
@@ -349,6 +410,12 @@ struct chp_ctx {
   bool worlds_dirty = true;
   int* d_list = nullptr;
   size_t d_list_cap = 0;
+  std::vector<int> h_list;  // host copy of d_list (the source of its last upload)
+  // the large-world tables of the current list: spans, SAP tiles (int2 each), world list indices
+  int* d_big = nullptr;
+  size_t d_big_cap = 0;
+  std::vector<int> h_big;
+  int big_nspans = 0, big_ntiles = 0, big_nw = 0, big_rounds = 0;
   int* d_status = nullptr;
   const int** d_cptrs = nullptr;
   size_t d_status_cap = 0;
@@ -368,7 +435,8 @@ struct chp_ctx {
     std::vector<int> list;
     int nbig = 0, max_nb = 0, bl = 0, bg = 0;
     double dt = 0;
-    const void *dq = nullptr, *dqcls = nullptr, *dlist = nullptr, *dworlds = nullptr;
+    const void *dq = nullptr, *dqcls = nullptr, *dlist = nullptr, *dworlds = nullptr, *dbig = nullptr;
+    std::array<int, 4> big{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } sg;
@@ -496,7 +564,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   fix.emplace_back((void**)(void*)&D.gsr, L.add<R>((size_t)4 * GS_GROUPS * row_cap));
   SCR(vw, float, 8 * nb);
   SCR(last_level, int, std::max(nb, row_cap + 1));
-  SCR(sort_keys, unsigned long long, np2);
+  SCR(sort_keys, unsigned long long, np2 + nb);  // the grid-wide SAP sort merges between two halves
   size_t o_cnt = L.add<int>(C_COUNT);
   fix.emplace_back((void**)(void*)&D.counters, o_cnt);
 #undef ARR
@@ -517,6 +585,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   S.bytes = L.size;
   ctx->in_use += L.size;
   S.nb = nb;
+  S.broadphase = D.broadphase;
   S.pair_cap = pair_cap; S.contact_cap = contact_cap; S.row_cap = row_cap;
   S.off_ldamp = o_ld; S.off_adamp = o_ad; S.off_counters = o_cnt;
   S.last_dt = -1;
@@ -686,6 +755,72 @@ static int ensure_queue(chp_ctx* ctx, const uint32_t* ids, uint32_t n) {
   return CHP_OK;
 }
 
+// uploads the world list of the next launches and the large-world tables that go with it (the
+// host copies stay alive in the context until the next upload)
+static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
+  const size_t n = list.size();
+  if (ctx->d_list_cap < n) {
+    if (ctx->d_list) (void)hipFree(ctx->d_list);
+    ctx->d_list = nullptr;
+    ctx->d_list_cap = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->d_list, n * sizeof(int)));
+    ctx->d_list_cap = n;
+  }
+  ctx->h_list = list;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, ctx->h_list.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<int> spans, tiles, bigw;
+  int max_tiles = 1;
+  for (size_t li = 0; li < n; li++) {
+    const WorldSlot& S = ctx->worlds[list[li]];
+    if (S.nb < BP_BIG) continue;
+    bigw.push_back((int)li);
+    for (int s0 = 0; s0 < S.nb; s0 += BP_SPAN) { spans.push_back((int)li); spans.push_back(s0); }
+    if (S.broadphase == CHP_BROADPHASE_SAP) {
+      for (int s0 = 0; s0 < S.nb; s0 += SAP_TILE) { tiles.push_back((int)li); tiles.push_back(s0); }
+      max_tiles = std::max(max_tiles, (S.nb + SAP_TILE - 1) / SAP_TILE);
+    }
+  }
+  ctx->big_nspans = (int)spans.size() / 2;
+  ctx->big_ntiles = (int)tiles.size() / 2;
+  ctx->big_nw = (int)bigw.size();
+  ctx->big_rounds = 0;
+  while ((1 << ctx->big_rounds) < max_tiles) ctx->big_rounds++;
+  ctx->h_big = spans;
+  ctx->h_big.insert(ctx->h_big.end(), tiles.begin(), tiles.end());
+  ctx->h_big.insert(ctx->h_big.end(), bigw.begin(), bigw.end());
+  if (ctx->h_big.empty()) return CHP_OK;
+  if (ctx->d_big_cap < ctx->h_big.size()) {
+    if (ctx->d_big) (void)hipFree(ctx->d_big);
+    ctx->d_big = nullptr;
+    ctx->d_big_cap = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->d_big, ctx->h_big.size() * sizeof(int)));
+    ctx->d_big_cap = ctx->h_big.size();
+  }
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_big, ctx->h_big.data(), ctx->h_big.size() * sizeof(int), hipMemcpyHostToDevice,
+                             ctx->stream));
+  return CHP_OK;
+}
+
+// the broadphase of the large worlds of the current list (after k_bp, which leaves them alone)
+template <typename R> static void launch_big_bp(chp_ctx* ctx, const int* d_list) {
+  if (ctx->big_nw == 0) return;
+  const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
+  hipStream_t st = ctx->stream;
+  const int2* spans = (const int2*)ctx->d_big;
+  const int2* tiles = spans + ctx->big_nspans;
+  const int* bigw = (const int*)(tiles + ctx->big_ntiles);
+  const int ns = ctx->big_nspans, rounds = ctx->big_rounds;
+  k_big_prestep<R><<<ns, 64, 0, st>>>(dw, d_list, spans);
+  if (ctx->big_ntiles) {
+    k_sap_tile<R><<<ctx->big_ntiles, 256, 0, st>>>(dw, d_list, tiles, rounds == 0);
+    for (int r = 0; r < rounds; r++)
+      k_sap_merge<R><<<ns, 64, 0, st>>>(dw, d_list, spans, SAP_TILE << r, r & 1, r == rounds - 1);
+  }
+  k_big_rows<R><<<ns, 256, 0, st>>>(dw, d_list, spans, 0);
+  k_big_scan<R><<<ctx->big_nw, 1024, 0, st>>>(dw, d_list, bigw);
+  k_big_rows<R><<<ns, 256, 0, st>>>(dw, d_list, spans, 1);
+}
+
 enum { K_BP = 0, K_NARROW, K_ROWS, K_LEVELS, K_GS, K_POST, K_NSTAGE };
 static const char* kStageNames[K_NSTAGE] = {"k_bp", "k_narrow", "k_rows", "k_levels", "k_gs", "k_post"};
 
@@ -707,6 +842,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     const size_t per_body = BpLds<R>::bytes(1);
     const int lds_nb = std::min(max_nb, (int)((ctx->max_lds - 4096) / per_body));
     k_bp<R><<<n, 256, BpLds<R>::bytes(lds_nb), st>>>(dw, d_list, lds_nb);
+    launch_big_bp<R>(ctx, d_list);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
   {
@@ -762,6 +898,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
   chp_ctx::StepGraph& g = ctx->sg;
   if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block &&
         g.bg == ctx->solve_block && g.dq == ctx->d_q && g.dqcls == ctx->d_qcls && g.dlist == ctx->d_list &&
+        g.dbig == ctx->d_big && g.big == std::array<int, 4>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds} &&
         g.dworlds == ctx->d_worlds)) {
     drop_graph(ctx);
     HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
@@ -779,6 +916,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
     g.valid = true; g.list = list; g.nbig = nbig; g.max_nb = max_nb; g.dt = dt;
     g.bl = ctx->levels_block; g.bg = ctx->solve_block;
     g.dq = ctx->d_q; g.dqcls = ctx->d_qcls; g.dlist = ctx->d_list; g.dworlds = ctx->d_worlds;
+    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds};
     g.graph = graph;
     g.exec = exec;
   }
@@ -794,17 +932,13 @@ template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, ui
   if (r != CHP_OK) return r;
   r = ensure_queue(ctx, ids, n);
   if (r != CHP_OK) return r;
-  if (ctx->d_list_cap < n) {
-    if (ctx->d_list) (void)hipFree(ctx->d_list);
-    HIPCHK(ctx, hipMalloc(&ctx->d_list, (size_t)n * sizeof(int)));
-    ctx->d_list_cap = n;
-  }
   // largest worlds first so that the long workgroups start early
   std::vector<int> list(ids, ids + n);
   std::stable_sort(list.begin(), list.end(), [&](int a, int b) { return ctx->worlds[a].nb > ctx->worlds[b].nb; });
   int nbig = 0;
   while (nbig < (int)n && ctx->worlds[list[nbig]].nb > SMALL_NB) nbig++;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, list.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  r = upload_list(ctx, list);
+  if (r != CHP_OK) return r;
   size_t nev = ctx->profile ? (size_t)nsteps * (K_NSTAGE + 1) : 0;
   while (ctx->sev.size() < nev) {
     hipEvent_t e;
@@ -878,12 +1012,8 @@ template <typename R> static int step_plain_t(chp_ctx* ctx, uint32_t id, double 
   if (r != CHP_OK) return r;
   r = ensure_queue(ctx, &id, 1);
   if (r != CHP_OK) return r;
-  if (ctx->d_list_cap < 1) {
-    HIPCHK(ctx, hipMalloc(&ctx->d_list, sizeof(int)));
-    ctx->d_list_cap = 1;
-  }
-  int one = (int)id;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  r = upload_list(ctx, {(int)id});
+  if (r != CHP_OK) return r;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int s = 0; s < nsteps; s++) {
     r = launch_step<R>(ctx, ctx->d_list, 1, ctx->worlds[id].nb > SMALL_NB ? 1 : 0, ctx->worlds[id].nb, dt, nullptr);
@@ -975,7 +1105,13 @@ template <typename R> static int state_io(chp_ctx* ctx, uint32_t id, chp_body_st
 template <typename R> static int broadphase_t(chp_ctx* ctx, uint32_t id, int32_t* pi, int32_t* pj, uint32_t cap, uint32_t* np) {
   int r = prepare<R>(ctx, &id, 1, ctx->worlds[id].last_dt > 0 ? ctx->worlds[id].last_dt : 1.0 / 60);
   if (r != CHP_OK) return r;
-  k_broadphase<R><<<1, CHP_BLOCK, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, (int)id);
+  if (ctx->worlds[id].nb >= BP_BIG) {
+    r = upload_list(ctx, {(int)id});
+    if (r != CHP_OK) return r;
+    launch_big_bp<R>(ctx, ctx->d_list);
+  } else {
+    k_broadphase<R><<<1, CHP_BLOCK, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, (int)id);
+  }
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   DevWorld<R>& D = dev_of<R>(ctx->worlds[id]);
@@ -1102,14 +1238,10 @@ static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t 
   if (r != CHP_OK) return r;
   r = ensure_eq(ctx, n);
   if (r != CHP_OK) return r;
-  if (ctx->d_list_cap < 1) {
-    HIPCHK(ctx, hipMalloc(&ctx->d_list, sizeof(int)));
-    ctx->d_list_cap = 1;
-  }
+  r = upload_list(ctx, {(int)id});
+  if (r != CHP_OK) return r;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (n) HIPCHK(ctx, hipMemcpy(ctx->d_eq, eqs, n * sizeof(chp_equation), hipMemcpyHostToDevice));
-  int one = (int)id;
-  HIPCHK(ctx, hipMemcpy(ctx->d_list, &one, sizeof(int), hipMemcpyHostToDevice));
   const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
   const int bl = ctx->levels_block, bg = ctx->solve_block, nbig = S.nb > SMALL_NB ? 1 : 0;
   hipStream_t st = ctx->stream;
@@ -1299,6 +1431,7 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   drop_graph(ctx);
   ctx->arena.destroy();
   if (ctx->d_worlds) (void)hipFree(ctx->d_worlds);
+  if (ctx->d_big) (void)hipFree(ctx->d_big);
   if (ctx->d_list) (void)hipFree(ctx->d_list);
   if (ctx->d_status) (void)hipFree(ctx->d_status);
   if (ctx->d_cptrs) (void)hipFree(ctx->d_cptrs);

@@ -106,26 +106,28 @@ template <typename R> __device__ void body_aabb(const DevWorld<R>& W, int i) {
   if (W.shape_count[i] > 0) { S3<R>(W.aabb_lo, i, lo); S3<R>(W.aabb_hi, i, hi); }
 }
 
-template <typename R> __device__ void st_prestep(const DevWorld<R>& W) {
-  const R gx = W.gravity[0], gy = W.gravity[1], gz = W.gravity[2];
-  for (int i = threadIdx.x; i < W.nb; i += blockDim.x) {
-    int ty = W.type[i];
-    f3 f = F3<R>(W.force, i);
-    if (ty == CHP_BODY_DYNAMIC) {
-      R m = W.mass[i];
-      f = set3<R>(ld<R>(f.x) + m * gx, ld<R>(f.y) + m * gy, ld<R>(f.z) + m * gz);
-    }
-    S3<R>(W.feff, i, f);
-    bool kin = ty == CHP_BODY_KINEMATIC;
-    W.ims[i] = kin ? (R)0 : W.inv_mass[i];
-    for (int k = 0; k < 9; k++) W.iiws[9 * i + k] = kin ? 0.f : W.iiw[9 * i + k];
-    S3<R>(W.vl, i, mk3(0.f, 0.f, 0.f));
-    S3<R>(W.wl, i, mk3(0.f, 0.f, 0.f));
-    if (W.broadphase == CHP_BROADPHASE_SAP && (W.flags[i] & BF_AABB_DIRTY)) {
-      body_aabb(W, i);
-      W.flags[i] &= ~BF_AABB_DIRTY;
-    }
+// World.internalStep before the broadphase for body i (world_class.dart:459-471): gravity into
+// the effective force, the solve masses, cleared velocity deltas, the SAP AABB refresh
+template <typename R> __device__ __forceinline__ void prestep_body(const DevWorld<R>& W, int i) {
+  int ty = W.type[i];
+  f3 f = F3<R>(W.force, i);
+  if (ty == CHP_BODY_DYNAMIC) {
+    R m = W.mass[i];
+    f = set3<R>(ld<R>(f.x) + m * (R)W.gravity[0], ld<R>(f.y) + m * (R)W.gravity[1], ld<R>(f.z) + m * (R)W.gravity[2]);
   }
+  S3<R>(W.feff, i, f);
+  bool kin = ty == CHP_BODY_KINEMATIC;
+  W.ims[i] = kin ? (R)0 : W.inv_mass[i];
+  for (int k = 0; k < 9; k++) W.iiws[9 * i + k] = kin ? 0.f : W.iiw[9 * i + k];
+  S3<R>(W.vl, i, mk3(0.f, 0.f, 0.f));
+  S3<R>(W.wl, i, mk3(0.f, 0.f, 0.f));
+  if (W.broadphase == CHP_BROADPHASE_SAP && (W.flags[i] & BF_AABB_DIRTY)) {
+    body_aabb(W, i);
+    W.flags[i] &= ~BF_AABB_DIRTY;
+  }
+}
+template <typename R> __device__ void st_prestep(const DevWorld<R>& W) {
+  for (int i = threadIdx.x; i < W.nb; i += blockDim.x) prestep_body(W, i);
 }
 
 // ---- broadphase -------------------------------------------------------------------------------
@@ -306,6 +308,129 @@ template <typename R> __device__ void st_filter(const DevWorld<R>& W) {
     }
   }
   __syncthreads();
+}
+
+// ---- broadphase of large worlds, spread over the grid -----------------------------------------
+// A world of at least BP_BIG bodies is not left to one workgroup.  Its rows are dealt out in spans
+// of BP_SPAN (one workgroup per span, one wave per row); the SAP axis list is sorted in tiles of
+// SAP_TILE keys (bitonic sort in LDS, one workgroup per tile) that are then merged pairwise by
+// rank, and the pairs are counted, scanned and written in three grid-wide passes.  Sort order,
+// pair order and the collideConnected filter are those of the workgroup path above.
+constexpr int BP_BIG = 1024, BP_SPAN = 64, SAP_TILE = 4096;
+
+template <typename R> __device__ __forceinline__ bool bp_filtered(const DevWorld<R>& W, int a, int b) {
+  for (int c = 0; c < W.ncons; c++) {
+    if (W.c_collide[c]) continue;
+    int ca = W.c_a[c], cb = W.c_b[c];
+    if ((ca == a && cb == b) || (cb == a && ca == b)) return true;
+  }
+  return false;
+}
+
+// SAP candidates m0..m0+63 of the body bi at sorted position k (A2 = its upper bound on the
+// axis): the hits before the first break (a pair that passes needBroadphaseCollision and fails
+// checkBounds); *stop when the sweep of this row ends in this chunk
+template <typename R>
+__device__ __forceinline__ unsigned long long sap_chunk(const DevWorld<R>& W, int bi, double A2, int m0, int* bj,
+                                                        bool* stop) {
+  const int n = W.nb, m = m0 + (int)(threadIdx.x & 63);
+  int j = m < n ? W.order[m] : 0;
+  bool need = m < n && bp_need(W, bi, j);
+  bool brk = need && !((double)axis_of(F3<R>(W.pos, j), W.sap_axis) - (double)W.brad[j] < A2);
+  bool hit = need && !brk && bp_sphere(W, bi, j);
+  unsigned long long bm = __ballot(brk), hm = __ballot(hit);
+  if (bm) hm &= (1ull << __builtin_ctzll(bm)) - 1ull;
+  *stop = bm != 0 || m0 + 64 >= n;
+  *bj = j;
+  return hm;
+}
+
+// row k of a large world (wave-uniform k): SAP, the body at sorted position k against the ones
+// after it; naive, body k against the bodies before it.  emit: write the pairs from offset o,
+// else return their number.
+template <typename R> __device__ int big_row(const DevWorld<R>& W, int k, bool emit, int o) {
+  const int lane = threadIdx.x & 63;
+  const bool filt = W.has_filtered_constraints;
+  int c = 0;
+  auto put = [&](unsigned long long m, int a, int b) {
+    if (emit && ((m >> lane) & 1ull)) {
+      int p = o + c + __popcll(m & ((1ull << lane) - 1ull));
+      W.pair_i[p] = (filt && bp_filtered(W, a, b)) ? -1 : a;
+      W.pair_j[p] = b;
+    }
+    c += __popcll(m);
+  };
+  if (W.broadphase == CHP_BROADPHASE_SAP) {
+    const int bi = W.order[k];
+    const double A2 = (double)axis_of(F3<R>(W.pos, bi), W.sap_axis) + (double)W.brad[bi];
+    for (int m0 = k + 1; m0 < W.nb; m0 += 64) {
+      int bj;
+      bool stop;
+      unsigned long long m = sap_chunk(W, bi, A2, m0, &bj, &stop);
+      put(m, bi, bj);
+      if (stop) break;
+    }
+  } else {
+    BpLds<R> L;
+    for (int j0 = 0; j0 < k; j0 += 64) put(bp_row_chunk<R, false>(W, L, k, j0), k, j0 + lane);
+  }
+  return c;
+}
+
+// one SAP tile: keys (order key of aabb.lowerBound[axis], position) of positions s0.. sorted in
+// LDS.  apply (the world is one tile): the new axis list is written at once; else the sorted run
+// goes to sort_keys and the old list to tmp for the merges.
+template <typename R>
+__device__ void sap_tile(const DevWorld<R>& W, int s0, unsigned long long* lk, int* lord, bool apply) {
+  const int cnt = min(SAP_TILE, W.nb - s0), ax = W.sap_axis;
+  for (int k = threadIdx.x; k < SAP_TILE; k += blockDim.x) {
+    unsigned long long key = ~0ull;
+    if (k < cnt) {
+      int g = s0 + k, b = W.order[g];
+      key = ((unsigned long long)order_key(axis_of(F3<R>(W.aabb_lo, b), ax)) << 32) | (unsigned)g;
+      if (apply) lord[k] = b;
+      else W.tmp[g] = b;
+    }
+    lk[k] = key;
+  }
+  __syncthreads();
+  for (int size = 2; size <= SAP_TILE; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < SAP_TILE / 2; t += blockDim.x) {
+        int lo = ((t & ~(stride - 1)) << 1) | (t & (stride - 1)), hi = lo + stride;
+        bool up = (lo & size) == 0;
+        unsigned long long a = lk[lo], b = lk[hi];
+        if ((a > b) == up) { lk[lo] = b; lk[hi] = a; }
+      }
+      __syncthreads();
+    }
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    if (apply) W.order[s0 + k] = lord[(int)(lk[k] & 0xffffffffu) - s0];
+    else W.sort_keys[s0 + k] = lk[k];
+  }
+}
+
+// merge round of run width w: the key at g goes to its run's start plus its index in the run plus
+// the number of keys below it in the partner run (keys are unique).  src selects the half of
+// sort_keys read; apply (last round) writes the new axis list.
+template <typename R> __device__ void sap_merge(const DevWorld<R>& W, int g, int w, int src, bool apply) {
+  const int n = W.nb;
+  const unsigned long long* a = W.sort_keys + (src ? n : 0);
+  unsigned long long* d = W.sort_keys + (src ? 0 : n);
+  const unsigned long long x = a[g];
+  const int run = g / w, base = run * w, pb = (run ^ 1) * w;
+  int out = g;
+  if (pb < n) {
+    int lo = pb, hi = min(pb + w, n);
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (a[mid] < x) lo = mid + 1;
+      else hi = mid;
+    }
+    out = min(base, pb) + (g - base) + (lo - pb);
+  }
+  d[out] = x;
+  if (apply) W.order[out] = W.tmp[(int)(x & 0xffffffffu)];
 }
 
 // ---- narrowphase ------------------------------------------------------------------------------
