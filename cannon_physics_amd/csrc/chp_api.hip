@@ -87,13 +87,17 @@ __global__ __launch_bounds__(64) void k_sap_merge(const DevWorld<R>* __restrict_
   const int g = sp.y + threadIdx.x;
   if (W.broadphase == CHP_BROADPHASE_SAP && g < W.nb) sap_merge(W, g, w, src, apply != 0);
 }
-// pass 0 counts the pairs of each row into cnt; pass 1 writes them from the scanned offsets
+// pass 0 counts the pairs of each row into cnt; pass 1 writes them from the scanned offsets.
+// Four workgroups per span, BP_ROWS rows each, one wave per row.
+constexpr int BP_ROWS = BP_SPAN / 4;
 template <typename R>
 __global__ __launch_bounds__(256) void k_big_rows(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
                                                   const int2* __restrict__ spans, int emit) {
-  CHP_SPAN_PROLOGUE(spans)
-  const int end = min(sp.y + BP_SPAN, W.nb);
-  for (int k = sp.y + (int)(threadIdx.x >> 6); k < end; k += 4) {
+  const int2 sp = spans[blockIdx.x >> 2];
+  const DevWorld<R>& W = worlds[list[sp.x]];
+  if (W.counters[C_STATUS] != 0) return;
+  const int r0 = sp.y + BP_ROWS * (int)(blockIdx.x & 3), end = min(r0 + BP_ROWS, W.nb);
+  for (int k = r0 + (int)(threadIdx.x >> 6); k < end; k += 4) {
     if (emit) {
       big_row(W, k, true, W.cnt[k]);
     } else {
@@ -108,11 +112,99 @@ __global__ __launch_bounds__(1024) void k_big_scan(const DevWorld<R>* __restrict
   __shared__ int sh[1024];
   const DevWorld<R>& W = worlds[list[bigw[blockIdx.x]]];
   if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
-  const int P = array_exscan(W.cnt, W.nb, sh);
+  const int P = array_exscan_chunked(W.cnt, W.nb, sh);
   if (threadIdx.x == 0) {
     W.counters[C_PAIRS] = P;
     if (P > W.pair_cap) ws_fail(W.counters, WS_PAIRS);
   }
+}
+
+// narrowphase class counts and queue placement of a large world's pairs: the world's spans are
+// its K workers, worker k takes pairs k * 256 + t, stride K * 256.  Counts add into the world's
+// qcls entries (zeroed by k_narrow_count); placement reserves each class range per workgroup.
+template <typename R>
+__device__ __forceinline__ void big_pair_classes(const DevWorld<R>& W, int2 sp, int* sh, bool mark_filtered) {
+  const int K = (W.nb + BP_SPAN - 1) / BP_SPAN, k = sp.y / BP_SPAN, P = W.counters[C_PAIRS];
+  for (int p = k * 256 + (int)threadIdx.x; p < P; p += K * 256) {
+    int bi = W.pair_i[p], bj = W.pair_j[p];
+    if (bi < 0) {
+      if (mark_filtered) pair_counts(W, p, 0, 0);
+      continue;
+    }
+    atomicAdd(&sh[pair_class(W, bi, bj)], 1);
+  }
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_big_ncount(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                    const int2* __restrict__ spans, int* __restrict__ qcls) {
+  __shared__ int sh[NP_NCLS];
+  CHP_SPAN_PROLOGUE(spans)
+  if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = 0;
+  __syncthreads();
+  big_pair_classes(W, sp, sh, true);
+  __syncthreads();
+  if (threadIdx.x < NP_NCLS && sh[threadIdx.x]) atomicAdd(&qcls[sp.x * NP_NCLS + threadIdx.x], sh[threadIdx.x]);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_big_nplace(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                    const int2* __restrict__ spans, int* __restrict__ qcls,
+                                                    int2* __restrict__ q) {
+  __shared__ int sh[NP_NCLS];
+  CHP_SPAN_PROLOGUE(spans)
+  if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = 0;
+  __syncthreads();
+  big_pair_classes(W, sp, sh, false);
+  __syncthreads();
+  if (threadIdx.x < NP_NCLS)
+    sh[threadIdx.x] = sh[threadIdx.x] ? atomicAdd(&qcls[sp.x * NP_NCLS + threadIdx.x], sh[threadIdx.x]) : 0;
+  __syncthreads();
+  const int K = (W.nb + BP_SPAN - 1) / BP_SPAN, k = sp.y / BP_SPAN, P = W.counters[C_PAIRS], wid = list[sp.x];
+  for (int p = k * 256 + (int)threadIdx.x; p < P; p += K * 256) {
+    int bi = W.pair_i[p], bj = W.pair_j[p];
+    if (bi < 0) continue;
+    q[atomicAdd(&sh[pair_class(W, bi, bj)], 1)] = make_int2(wid, p);
+  }
+}
+
+// rows of a large world: the offsets and counters in one workgroup per world, then the rows of
+// its pairs and constraints over its spans (worker k of K as above)
+template <typename R>
+__global__ __launch_bounds__(1024) void k_big_rowscan(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                      const int* __restrict__ bigw) {
+  __shared__ int sh[1024];
+  const DevWorld<R>& W = worlds[list[bigw[blockIdx.x]]];
+  if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
+  const int P = W.counters[C_PAIRS];
+  const int F = array_exscan_chunked(W.pf_cnt, P, sh);
+  const int C = array_exscan_chunked(W.pc_off, P, sh);
+  const int E = F + C + W.ncrows;
+  if (threadIdx.x == 0) {
+    if (E > W.row_cap) {
+      W.counters[C_ROWS] = E;
+      ws_fail(W.counters, WS_ROWS);
+    } else {
+      rows_done(W, P, F, C, E);
+    }
+  }
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_big_rowbuild(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                      const int2* __restrict__ spans, double dt) {
+  CHP_SPAN_PROLOGUE(spans)
+  const int K = (W.nb + BP_SPAN - 1) / BP_SPAN, k = sp.y / BP_SPAN, P = W.counters[C_PAIRS];
+  const int F = W.counters[C_FROWS], C = W.counters[C_CROWS];
+  for (int p = k * 256 + (int)threadIdx.x; p < P; p += K * 256) pair_rows(W, p, F, (R)dt);
+  for (int c = k * 256 + (int)threadIdx.x; c < W.ncons; c += K * 256) constraint_rows(W, c, F + C + W.c_row_off[c], (R)dt);
+}
+
+// integration of a large world's bodies over its spans; a non-finite pose is noted in
+// C_POSTFAIL and raised by k_post, which also advances the step counter
+template <typename R>
+__global__ __launch_bounds__(64) void k_big_post(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                 const int2* __restrict__ spans, double dt) {
+  CHP_SPAN_PROLOGUE(spans)
+  const int i = sp.y + threadIdx.x;
+  if (i < W.nb && !post_body(W, i, (R)dt, W.counters[C_ROWS], post_qn(W))) W.counters[C_POSTFAIL] = 1;
 }
 
 // narrowphase: per-class pair counts per world, class-major offsets, placement in the queue,
@@ -125,7 +217,7 @@ __global__ __launch_bounds__(64) void k_narrow_count(const DevWorld<R>* __restri
   __shared__ int sh[64];
   int* cnt = qcls + (size_t)blockIdx.x * NP_NCLS;
   const DevWorld<R>& W = worlds[list[blockIdx.x]];
-  if (block_bcast(W.counters[C_STATUS], sh) != 0) {
+  if (block_bcast(W.counters[C_STATUS], sh) != 0 || W.nb >= BP_BIG) {  // large worlds: k_big_ncount
     if (threadIdx.x < NP_NCLS) cnt[threadIdx.x] = 0;
     return;
   }
@@ -140,6 +232,7 @@ __global__ __launch_bounds__(64) void k_narrow(const DevWorld<R>* __restrict__ w
                                                const int* __restrict__ qcls, int2* __restrict__ q) {
   __shared__ int sh[64];
   CHP_WORLD_PROLOGUE
+  if (W.nb >= BP_BIG) return;  // k_big_nplace
   st_narrow_place(W, list[blockIdx.x], qcls + (size_t)blockIdx.x * NP_NCLS, q, sh);
 }
 template <typename R>
@@ -160,6 +253,7 @@ __global__ __launch_bounds__(128) void k_rows(const DevWorld<R>* __restrict__ wo
                                               double dt) {
   __shared__ int sh[128];
   CHP_WORLD_PROLOGUE
+  if (W.nb >= BP_BIG) return;  // k_big_rowscan / k_big_rowbuild
   st_rows(W, (R)dt, sh);
 }
 
@@ -249,6 +343,16 @@ __global__ __launch_bounds__(256) void k_post(const DevWorld<R>* __restrict__ wo
                                               double dt) {
   __shared__ int sh[256];
   CHP_WORLD_PROLOGUE
+  if (W.nb >= BP_BIG) {  // k_big_post integrated the bodies
+    if (threadIdx.x == 0) {
+      if (W.counters[C_POSTFAIL]) {
+        W.counters[C_POSTFAIL] = 0;
+        ws_fail(W.counters, WS_NONFINITE);
+      }
+      W.counters[C_STEP] += 1;
+    }
+    return;
+  }
   st_post(W, (R)dt);
 }
 
@@ -565,6 +669,8 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   SCR(vw, float, 8 * nb);
   SCR(last_level, int, std::max(nb, row_cap + 1));
   SCR(sort_keys, unsigned long long, np2 + nb);  // the grid-wide SAP sort merges between two halves
+  const int nsr = (H.wd.broadphase == CHP_BROADPHASE_SAP && nb >= BP_BIG) ? nb : 0;
+  SCR(sbp, float, 4 * nsr); SCR(sgm, int, 4 * nsr); SCR(sbr, R, nsr);
   size_t o_cnt = L.add<int>(C_COUNT);
   fix.emplace_back((void**)(void*)&D.counters, o_cnt);
 #undef ARR
@@ -801,6 +907,11 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   return CHP_OK;
 }
 
+static const int2* big_spans(const chp_ctx* ctx) { return (const int2*)ctx->d_big; }
+static const int* big_worlds(const chp_ctx* ctx) {
+  return (const int*)((const int2*)ctx->d_big + ctx->big_nspans + ctx->big_ntiles);
+}
+
 // the broadphase of the large worlds of the current list (after k_bp, which leaves them alone)
 template <typename R> static void launch_big_bp(chp_ctx* ctx, const int* d_list) {
   if (ctx->big_nw == 0) return;
@@ -816,9 +927,9 @@ template <typename R> static void launch_big_bp(chp_ctx* ctx, const int* d_list)
     for (int r = 0; r < rounds; r++)
       k_sap_merge<R><<<ns, 64, 0, st>>>(dw, d_list, spans, SAP_TILE << r, r & 1, r == rounds - 1);
   }
-  k_big_rows<R><<<ns, 256, 0, st>>>(dw, d_list, spans, 0);
+  k_big_rows<R><<<4 * ns, 256, 0, st>>>(dw, d_list, spans, 0);
   k_big_scan<R><<<ctx->big_nw, 1024, 0, st>>>(dw, d_list, bigw);
-  k_big_rows<R><<<ns, 256, 0, st>>>(dw, d_list, spans, 1);
+  k_big_rows<R><<<4 * ns, 256, 0, st>>>(dw, d_list, spans, 1);
 }
 
 enum { K_BP = 0, K_NARROW, K_ROWS, K_LEVELS, K_GS, K_POST, K_NSTAGE };
@@ -849,8 +960,11 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     const int qcap = (int)ctx->d_q_cap;
     int* bounds = ctx->d_qcls + (size_t)n * NP_NCLS;
     k_narrow_count<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls);
+    if (ctx->big_nw) k_big_ncount<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls);
     k_narrow_scan<<<1, 1024, 0, st>>>(ctx->d_qcls, n, bounds);
     k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls, ctx->d_q);
+    if (ctx->big_nw)
+      k_big_nplace<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls, ctx->d_q);
     const int threads = std::max(64, std::min(qcap, 256 * 4 * 64));
     k_narrow_thread<R><<<(threads + 63) / 64, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
     const int waves = std::max(1, std::min(qcap, 4096));
@@ -858,6 +972,10 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
   k_rows<R><<<n, 128, 0, st>>>(dw, d_list, dt);
+  if (ctx->big_nw) {
+    k_big_rowscan<R><<<ctx->big_nw, 1024, 0, st>>>(dw, d_list, big_worlds(ctx));
+    k_big_rowbuild<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), dt);
+  }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
   if (tune) {
     for (int c = 0; c < 3; c++) {
@@ -879,6 +997,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     k_gs<R><<<groups(bg), bg, lds_gs(bg), st>>>(dw, d_list, nbig, n);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
   }
+  if (ctx->big_nw) k_big_post<R><<<ctx->big_nspans, 64, 0, st>>>(dw, d_list, big_spans(ctx), dt);
   k_post<R><<<n, 256, 0, st>>>(dw, d_list, dt);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
   HIPCHK(ctx, hipGetLastError());

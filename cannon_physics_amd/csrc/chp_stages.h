@@ -11,6 +11,7 @@ namespace chp {
 
 enum { C_STATUS = 0, C_PAIRS = 1, C_CONTACTS = 2, C_ROWS = 3, C_ITERS = 4, C_LEVELS = 5, C_STAGING = 6,
        C_STEP = 7, C_FROWS = 8, C_CROWS = 9, C_SUM_PAIRS = 10, C_SUM_ROWS = 11, C_SUM_CONTACTS = 12,
+       C_POSTFAIL = 13,
        C_COUNT = 16 };
 
 __device__ __forceinline__ void ws_fail(int* counters, int bit) { atomicOr(&counters[C_STATUS], bit); }
@@ -44,6 +45,23 @@ __device__ __forceinline__ int array_exscan(int* data, int n, int* sh) {
   }
   __syncthreads();
   return carry;
+}
+// the same for long arrays: each thread scans a contiguous chunk (two passes over the data, one
+// block-wide scan of the chunk sums)
+__device__ __forceinline__ int array_exscan_chunked(int* data, int n, int* sh) {
+  const int c = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int b = min(n, (int)threadIdx.x * c), e = min(n, b + c);
+  int s = 0;
+  for (int i = b; i < e; i++) s += data[i];
+  int tot;
+  int ex = block_exscan(s, sh, &tot);
+  for (int i = b; i < e; i++) {
+    int v = data[i];
+    data[i] = ex;
+    ex += v;
+  }
+  __syncthreads();
+  return tot;
 }
 template <typename R> __device__ __forceinline__ R block_sum(R v, R* sh) {
   int t = threadIdx.x;
@@ -138,10 +156,13 @@ template <typename R> __device__ __forceinline__ bool bp_need(const DevWorld<R>&
   if (W.type[a] == CHP_BODY_STATIC && W.type[b] == CHP_BODY_STATIC) return false;
   return true;
 }
-template <typename R> __device__ __forceinline__ bool bp_sphere(const DevWorld<R>& W, int a, int b) {
-  f3 r = sub3<double>(F3<R>(W.pos, b), F3<R>(W.pos, a));
-  double s = (double)W.brad[a] + (double)W.brad[b];
+__device__ __forceinline__ bool bp_sphere_at(f3 pa, double ra, f3 pb, double rb) {
+  f3 r = sub3<double>(pb, pa);
+  double s = ra + rb;
   return len2<double>(r) < s * s;
+}
+template <typename R> __device__ __forceinline__ bool bp_sphere(const DevWorld<R>& W, int a, int b) {
+  return bp_sphere_at(F3<R>(W.pos, a), (double)W.brad[a], F3<R>(W.pos, b), (double)W.brad[b]);
 }
 __device__ __forceinline__ float axis_of(f3 v, int ax) { return ax == 0 ? v.x : (ax == 1 ? v.y : v.z); }
 
@@ -327,21 +348,37 @@ template <typename R> __device__ __forceinline__ bool bp_filtered(const DevWorld
   return false;
 }
 
-// SAP candidates m0..m0+63 of the body bi at sorted position k (A2 = its upper bound on the
-// axis): the hits before the first break (a pair that passes needBroadphaseCollision and fails
-// checkBounds); *stop when the sweep of this row ends in this chunk
+// the sweep record of body b at axis-list position k
+template <typename R> __device__ __forceinline__ void sap_record(const DevWorld<R>& W, int k, int b) {
+  const f3 p = F3<R>(W.pos, b);
+  ((float4*)W.sbp)[k] = make_float4(p.x, p.y, p.z, __int_as_float(b));
+  ((int4*)W.sgm)[k] = make_int4(W.group[b], W.mask[b], W.type[b] == CHP_BODY_STATIC ? 1 : 0, 0);
+  W.sbr[k] = W.brad[b];
+}
+
+// SAP candidates m0..m0+63 of the row body (filter gi, position pi, radius ri, upper bound A2 on
+// the axis), read from the sweep records: the hits before the first break (a pair that passes
+// needBroadphaseCollision and fails checkBounds); *stop when the row's sweep ends in this chunk
 template <typename R>
-__device__ __forceinline__ unsigned long long sap_chunk(const DevWorld<R>& W, int bi, double A2, int m0, int* bj,
-                                                        bool* stop) {
+__device__ __forceinline__ unsigned long long sap_chunk(const DevWorld<R>& W, int4 gi, f3 pi, double ri, double A2,
+                                                        int m0, int* bj, bool* stop) {
   const int n = W.nb, m = m0 + (int)(threadIdx.x & 63);
-  int j = m < n ? W.order[m] : 0;
-  bool need = m < n && bp_need(W, bi, j);
-  bool brk = need && !((double)axis_of(F3<R>(W.pos, j), W.sap_axis) - (double)W.brad[j] < A2);
-  bool hit = need && !brk && bp_sphere(W, bi, j);
+  float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+  int4 gj = make_int4(0, 0, 0, 0);
+  double rj = 0.0;
+  if (m < n) {
+    pj = ((const float4*)W.sbp)[m];
+    gj = ((const int4*)W.sgm)[m];
+    rj = (double)W.sbr[m];
+  }
+  const f3 pv = mk3(pj.x, pj.y, pj.z);
+  bool need = m < n && (gi.x & gj.y) != 0 && (gj.x & gi.y) != 0 && !(gi.z && gj.z);
+  bool brk = need && !((double)axis_of(pv, W.sap_axis) - rj < A2);
+  bool hit = need && !brk && bp_sphere_at(pi, ri, pv, rj);
   unsigned long long bm = __ballot(brk), hm = __ballot(hit);
   if (bm) hm &= (1ull << __builtin_ctzll(bm)) - 1ull;
   *stop = bm != 0 || m0 + 64 >= n;
-  *bj = j;
+  *bj = __float_as_int(pj.w);
   return hm;
 }
 
@@ -361,12 +398,16 @@ template <typename R> __device__ int big_row(const DevWorld<R>& W, int k, bool e
     c += __popcll(m);
   };
   if (W.broadphase == CHP_BROADPHASE_SAP) {
-    const int bi = W.order[k];
-    const double A2 = (double)axis_of(F3<R>(W.pos, bi), W.sap_axis) + (double)W.brad[bi];
+    const float4 p4 = ((const float4*)W.sbp)[k];
+    const int4 gi = ((const int4*)W.sgm)[k];
+    const int bi = __float_as_int(p4.w);
+    const f3 pi = mk3(p4.x, p4.y, p4.z);
+    const double ri = (double)W.sbr[k];
+    const double A2 = (double)axis_of(pi, W.sap_axis) + ri;
     for (int m0 = k + 1; m0 < W.nb; m0 += 64) {
       int bj;
       bool stop;
-      unsigned long long m = sap_chunk(W, bi, A2, m0, &bj, &stop);
+      unsigned long long m = sap_chunk(W, gi, pi, ri, A2, m0, &bj, &stop);
       put(m, bi, bj);
       if (stop) break;
     }
@@ -405,8 +446,13 @@ __device__ void sap_tile(const DevWorld<R>& W, int s0, unsigned long long* lk, i
       __syncthreads();
     }
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-    if (apply) W.order[s0 + k] = lord[(int)(lk[k] & 0xffffffffu) - s0];
-    else W.sort_keys[s0 + k] = lk[k];
+    if (apply) {
+      const int b = lord[(int)(lk[k] & 0xffffffffu) - s0];
+      W.order[s0 + k] = b;
+      sap_record(W, s0 + k, b);
+    } else {
+      W.sort_keys[s0 + k] = lk[k];
+    }
   }
 }
 
@@ -430,7 +476,11 @@ template <typename R> __device__ void sap_merge(const DevWorld<R>& W, int g, int
     out = min(base, pb) + (g - base) + (lo - pb);
   }
   d[out] = x;
-  if (apply) W.order[out] = W.tmp[(int)(x & 0xffffffffu)];
+  if (apply) {
+    const int b = W.tmp[(int)(x & 0xffffffffu)];
+    W.order[out] = b;
+    sap_record(W, out, b);
+  }
 }
 
 // ---- narrowphase ------------------------------------------------------------------------------
@@ -743,6 +793,38 @@ template <typename R> __device__ void constraint_rows(const DevWorld<R>& W, int 
   }
 }
 
+// the friction rows (from the scanned offset pf_cnt[p]) and contact rows (from F + pc_off[p]) of
+// pair p's contacts
+template <typename R> __device__ void pair_rows(const DevWorld<R>& W, int p, int F, R h) {
+  int fo = W.pf_cnt[p], co = F + W.pc_off[p];
+  for (int k = 0; k < W.pc_cnt[p]; k++) {
+    const ContactRec<R>& c = W.contacts[W.pc_base[p] + k];
+    if (!(c.flags & 1)) continue;
+    const R* cm = W.cm_params + 6 * c.cm;
+    if (c.flags & 2) {
+      R fa, fb, fe;
+      spook<R>(cm[4], cm[5], h, fa, fb, fe);
+      f3 t1, t2;
+      tangents3<R>(c.ni, t1, t2);
+      friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t1, fb, fe, h, -c.slip, c.slip);
+      friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t2, fb, fe, h, -c.slip, c.slip);
+    }
+    R ca, cb, ce;
+    spook<R>(cm[2], cm[3], h, ca, cb, ce);
+    contact_row(W, co++, c.bi, c.bj, c.ri, c.rj, c.ni, c.restitution, ca, cb, ce, h, (R)0, (R)1e6);
+  }
+}
+// the row counters of a step (thread 0), after the offsets are known
+template <typename R> __device__ void rows_done(const DevWorld<R>& W, int P, int F, int C, int E) {
+  W.counters[C_ROWS] = E;
+  W.counters[C_FROWS] = F;
+  W.counters[C_CROWS] = C;
+  W.counters[C_CONTACTS] = W.counters[C_STAGING];
+  W.counters[C_SUM_PAIRS] += P;
+  W.counters[C_SUM_ROWS] += E;
+  W.counters[C_SUM_CONTACTS] += W.counters[C_STAGING];
+}
+
 // rows in solver order: every friction row, every contact row, then constraint rows
 // (world_class.dart:539-635; Solver.addEquation filter, solver.dart:30-34)
 template <typename R> __device__ bool st_rows(const DevWorld<R>& W, R h, int* sh) {
@@ -754,35 +836,9 @@ template <typename R> __device__ bool st_rows(const DevWorld<R>& W, R h, int* sh
     if (threadIdx.x == 0) { W.counters[C_ROWS] = E; ws_fail(W.counters, WS_ROWS); }
     return false;
   }
-  for (int p = threadIdx.x; p < P; p += blockDim.x) {
-    int fo = W.pf_cnt[p], co = F + W.pc_off[p];
-    for (int k = 0; k < W.pc_cnt[p]; k++) {
-      const ContactRec<R>& c = W.contacts[W.pc_base[p] + k];
-      if (!(c.flags & 1)) continue;
-      const R* cm = W.cm_params + 6 * c.cm;
-      if (c.flags & 2) {
-        R fa, fb, fe;
-        spook<R>(cm[4], cm[5], h, fa, fb, fe);
-        f3 t1, t2;
-        tangents3<R>(c.ni, t1, t2);
-        friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t1, fb, fe, h, -c.slip, c.slip);
-        friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t2, fb, fe, h, -c.slip, c.slip);
-      }
-      R ca, cb, ce;
-      spook<R>(cm[2], cm[3], h, ca, cb, ce);
-      contact_row(W, co++, c.bi, c.bj, c.ri, c.rj, c.ni, c.restitution, ca, cb, ce, h, (R)0, (R)1e6);
-    }
-  }
+  for (int p = threadIdx.x; p < P; p += blockDim.x) pair_rows(W, p, F, h);
   for (int c = threadIdx.x; c < W.ncons; c += blockDim.x) constraint_rows(W, c, F + C + W.c_row_off[c], h);
-  if (threadIdx.x == 0) {
-    W.counters[C_ROWS] = E;
-    W.counters[C_FROWS] = F;
-    W.counters[C_CROWS] = C;
-    W.counters[C_CONTACTS] = W.counters[C_STAGING];
-    W.counters[C_SUM_PAIRS] += P;
-    W.counters[C_SUM_ROWS] += E;
-    W.counters[C_SUM_CONTACTS] += W.counters[C_STAGING];
-  }
+  if (threadIdx.x == 0) rows_done(W, P, F, C, E);
   __syncthreads();
   return true;
 }
@@ -1332,71 +1388,80 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
   g.sync();
 }
 
+// body i's velocity update, damping and integration; false when its pose turned non-finite
+template <typename R> __device__ bool post_body(const DevWorld<R>& W, int i, R dt, int E, bool qn) {
+  bool ok = true;
+  f3 v = F3<R>(W.vel, i), w = F3<R>(W.avel, i);
+  f3 lf = F3<R>(W.lin_factor, i), af = F3<R>(W.ang_factor, i);
+  if (E != 0) {
+    v = add3<R>(v, mul3<R>(F3<R>(W.vl, i), lf));
+    w = add3<R>(w, mul3<R>(F3<R>(W.wl, i), af));
+  }
+  int ty = W.type[i];
+  if (ty == CHP_BODY_DYNAMIC) {
+    v = scale3<R>(v, W.ldamp[i]);
+    w = scale3<R>(w, W.adamp[i]);
+  }
+  if (ty == CHP_BODY_DYNAMIC || ty == CHP_BODY_KINEMATIC) {
+    f3 f = F3<R>(W.feff, i), tq = F3<R>(W.torque, i);
+    R iMdt = W.inv_mass[i] * dt;
+    v = set3<R>(ld<R>(v.x) + ld<R>(f.x) * iMdt * ld<R>(lf.x), ld<R>(v.y) + ld<R>(f.y) * iMdt * ld<R>(lf.y),
+                ld<R>(v.z) + ld<R>(f.z) * iMdt * ld<R>(lf.z));
+    const float* e = W.iiw + 9 * i;
+    R tx = ld<R>(tq.x) * ld<R>(af.x), ty_ = ld<R>(tq.y) * ld<R>(af.y), tz = ld<R>(tq.z) * ld<R>(af.z);
+    w = set3<R>(ld<R>(w.x) + dt * (ld<R>(e[0]) * tx + ld<R>(e[1]) * ty_ + ld<R>(e[2]) * tz),
+                ld<R>(w.y) + dt * (ld<R>(e[3]) * tx + ld<R>(e[4]) * ty_ + ld<R>(e[5]) * tz),
+                ld<R>(w.z) + dt * (ld<R>(e[6]) * tx + ld<R>(e[7]) * ty_ + ld<R>(e[8]) * tz));
+    f3 p = F3<R>(W.pos, i);
+    p = set3<R>(ld<R>(p.x) + ld<R>(v.x) * dt, ld<R>(p.y) + ld<R>(v.y) * dt, ld<R>(p.z) + ld<R>(v.z) * dt);
+    f4 q = ((const f4*)W.quat)[i];
+    {  // Quat.integrate (quaternion.dart:93-111)
+      R ax = ld<R>(w.x) * ld<R>(af.x), ay = ld<R>(w.y) * ld<R>(af.y), az = ld<R>(w.z) * ld<R>(af.z);
+      R bx = q.x, by = q.y, bz = q.z, bw = q.w;
+      R hd = dt * (R)0.5;
+      q.x = st<R>(bx + hd * (ax * bw + ay * bz - az * by));
+      q.y = st<R>(by + hd * (ay * bw + az * bx - ax * bz));
+      q.z = st<R>(bz + hd * (az * bw + ax * by - ay * bx));
+      q.w = st<R>(bw + hd * (-ax * bx - ay * by - az * bz));
+    }
+    if (qn) {
+      R x = q.x, y = q.y, z = q.z, ww = q.w;
+      if (W.quat_fast) {  // Quat.normalizeFast (quaternion.dart:171-185)
+        R f2 = ((R)3 - (x * x + y * y + z * z + ww * ww)) / (R)2;
+        if (f2 == (R)0) { q.x = q.y = q.z = q.w = 0.f; }
+        else { q.x = st<R>(x * f2); q.y = st<R>(y * f2); q.z = st<R>(z * f2); q.w = st<R>(ww * f2); }
+      } else {  // vector_math Quaternion.normalize
+        R l = sqrt((x * x) + (y * y) + (z * z) + (ww * ww));
+        if (l != (R)0) {
+          R d = (R)1 / l;
+          q.x = st<R>(x * d); q.y = st<R>(y * d); q.z = st<R>(z * d); q.w = st<R>(ww * d);
+        }
+      }
+    }
+    S3<R>(W.pos, i, p);
+    ((f4*)W.quat)[i] = q;
+    W.flags[i] |= BF_AABB_DIRTY;
+    if (!(W.flags[i] & BF_ISOTROPIC)) inertia_world<R>(q, F3<R>(W.inv_inertia, i), W.iiw + 9 * i);
+    ok = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(q.w);
+  }
+  S3<R>(W.vel, i, v);
+  S3<R>(W.avel, i, w);
+  S3<R>(W.force, i, mk3(0.f, 0.f, 0.f));   // World.clearForces
+  S3<R>(W.torque, i, mk3(0.f, 0.f, 0.f));
+  return ok;
+}
+// quaternion normalisation this step (World.quatNormalizeSkip)
+template <typename R> __device__ __forceinline__ bool post_qn(const DevWorld<R>& W) {
+  return W.counters[C_STEP] % (W.quat_skip + 1) == 0;
+}
+
 // ---- velocities, damping, integration ---------------------------------------------------------
 // gs_solver.dart:157-168, world_class.dart:647-674, Body.integrate (rigid_body.dart:627-680)
 template <typename R> __device__ void st_post(const DevWorld<R>& W, R dt) {
-  int E = W.counters[C_ROWS];
-  int step = W.counters[C_STEP];
-  bool qn = step % (W.quat_skip + 1) == 0;
-  for (int i = threadIdx.x; i < W.nb; i += blockDim.x) {
-    f3 v = F3<R>(W.vel, i), w = F3<R>(W.avel, i);
-    f3 lf = F3<R>(W.lin_factor, i), af = F3<R>(W.ang_factor, i);
-    if (E != 0) {
-      v = add3<R>(v, mul3<R>(F3<R>(W.vl, i), lf));
-      w = add3<R>(w, mul3<R>(F3<R>(W.wl, i), af));
-    }
-    int ty = W.type[i];
-    if (ty == CHP_BODY_DYNAMIC) {
-      v = scale3<R>(v, W.ldamp[i]);
-      w = scale3<R>(w, W.adamp[i]);
-    }
-    if (ty == CHP_BODY_DYNAMIC || ty == CHP_BODY_KINEMATIC) {
-      f3 f = F3<R>(W.feff, i), tq = F3<R>(W.torque, i);
-      R iMdt = W.inv_mass[i] * dt;
-      v = set3<R>(ld<R>(v.x) + ld<R>(f.x) * iMdt * ld<R>(lf.x), ld<R>(v.y) + ld<R>(f.y) * iMdt * ld<R>(lf.y),
-                  ld<R>(v.z) + ld<R>(f.z) * iMdt * ld<R>(lf.z));
-      const float* e = W.iiw + 9 * i;
-      R tx = ld<R>(tq.x) * ld<R>(af.x), ty_ = ld<R>(tq.y) * ld<R>(af.y), tz = ld<R>(tq.z) * ld<R>(af.z);
-      w = set3<R>(ld<R>(w.x) + dt * (ld<R>(e[0]) * tx + ld<R>(e[1]) * ty_ + ld<R>(e[2]) * tz),
-                  ld<R>(w.y) + dt * (ld<R>(e[3]) * tx + ld<R>(e[4]) * ty_ + ld<R>(e[5]) * tz),
-                  ld<R>(w.z) + dt * (ld<R>(e[6]) * tx + ld<R>(e[7]) * ty_ + ld<R>(e[8]) * tz));
-      f3 p = F3<R>(W.pos, i);
-      p = set3<R>(ld<R>(p.x) + ld<R>(v.x) * dt, ld<R>(p.y) + ld<R>(v.y) * dt, ld<R>(p.z) + ld<R>(v.z) * dt);
-      f4 q = ((const f4*)W.quat)[i];
-      {  // Quat.integrate (quaternion.dart:93-111)
-        R ax = ld<R>(w.x) * ld<R>(af.x), ay = ld<R>(w.y) * ld<R>(af.y), az = ld<R>(w.z) * ld<R>(af.z);
-        R bx = q.x, by = q.y, bz = q.z, bw = q.w;
-        R hd = dt * (R)0.5;
-        q.x = st<R>(bx + hd * (ax * bw + ay * bz - az * by));
-        q.y = st<R>(by + hd * (ay * bw + az * bx - ax * bz));
-        q.z = st<R>(bz + hd * (az * bw + ax * by - ay * bx));
-        q.w = st<R>(bw + hd * (-ax * bx - ay * by - az * bz));
-      }
-      if (qn) {
-        R x = q.x, y = q.y, z = q.z, ww = q.w;
-        if (W.quat_fast) {  // Quat.normalizeFast (quaternion.dart:171-185)
-          R f2 = ((R)3 - (x * x + y * y + z * z + ww * ww)) / (R)2;
-          if (f2 == (R)0) { q.x = q.y = q.z = q.w = 0.f; }
-          else { q.x = st<R>(x * f2); q.y = st<R>(y * f2); q.z = st<R>(z * f2); q.w = st<R>(ww * f2); }
-        } else {  // vector_math Quaternion.normalize
-          R l = sqrt((x * x) + (y * y) + (z * z) + (ww * ww));
-          if (l != (R)0) {
-            R d = (R)1 / l;
-            q.x = st<R>(x * d); q.y = st<R>(y * d); q.z = st<R>(z * d); q.w = st<R>(ww * d);
-          }
-        }
-      }
-      S3<R>(W.pos, i, p);
-      ((f4*)W.quat)[i] = q;
-      W.flags[i] |= BF_AABB_DIRTY;
-      if (!(W.flags[i] & BF_ISOTROPIC)) inertia_world<R>(q, F3<R>(W.inv_inertia, i), W.iiw + 9 * i);
-      if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(q.w))) ws_fail(W.counters, WS_NONFINITE);
-    }
-    S3<R>(W.vel, i, v);
-    S3<R>(W.avel, i, w);
-    S3<R>(W.force, i, mk3(0.f, 0.f, 0.f));   // World.clearForces
-    S3<R>(W.torque, i, mk3(0.f, 0.f, 0.f));
-  }
+  const int E = W.counters[C_ROWS], step = W.counters[C_STEP];
+  const bool qn = post_qn(W);
+  for (int i = threadIdx.x; i < W.nb; i += blockDim.x)
+    if (!post_body(W, i, dt, E, qn)) ws_fail(W.counters, WS_NONFINITE);
   __syncthreads();
   if (threadIdx.x == 0) W.counters[C_STEP] = step + 1;
   __syncthreads();

@@ -104,6 +104,11 @@ template <typename R> struct DevWorld {
   CHP_G(void) gsr;   // GS records in level order (24 R-sized words per row)
   CHP_G(float) vw;   // vlambda / wlambda, 8 floats per body, when the world is too big for LDS
   CHP_G(unsigned long long) sort_keys;
+  // SAP worlds of at least 1024 bodies: the sweep's body data in axis-list order (position and
+  // body index, 4 words; filter group, mask, static flag, 4 words; bounding radius)
+  CHP_G(float) sbp;
+  CHP_G(int) sgm;
+  CHP_G(R) sbr;
   CHP_G(int) counters;   // [0] status [1] pairs [2] contacts [3] rows [4] iterations [5] levels [6] staging
 };
 
