@@ -112,7 +112,7 @@ __global__ __launch_bounds__(1024) void k_big_scan(const DevWorld<R>* __restrict
   __shared__ int sh[1024];
   const DevWorld<R>& W = worlds[list[bigw[blockIdx.x]]];
   if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
-  const int P = array_exscan_chunked(W.cnt, W.nb, sh);
+  const int P = array_exscan(W.cnt, W.nb, sh);
   if (threadIdx.x == 0) {
     W.counters[C_PAIRS] = P;
     if (P > W.pair_cap) ws_fail(W.counters, WS_PAIRS);
@@ -175,8 +175,8 @@ __global__ __launch_bounds__(1024) void k_big_rowscan(const DevWorld<R>* __restr
   const DevWorld<R>& W = worlds[list[bigw[blockIdx.x]]];
   if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
   const int P = W.counters[C_PAIRS];
-  const int F = array_exscan_chunked(W.pf_cnt, P, sh);
-  const int C = array_exscan_chunked(W.pc_off, P, sh);
+  const int F = array_exscan(W.pf_cnt, P, sh);
+  const int C = array_exscan(W.pc_off, P, sh);
   const int E = F + C + W.ncrows;
   if (threadIdx.x == 0) {
     if (E > W.row_cap) {
@@ -235,10 +235,10 @@ __global__ __launch_bounds__(64) void k_narrow(const DevWorld<R>* __restrict__ w
   if (W.nb >= BP_BIG) return;  // k_big_nplace
   st_narrow_place(W, list[blockIdx.x], qcls + (size_t)blockIdx.x * NP_NCLS, q, sh);
 }
-template <typename R>
-__global__ __launch_bounds__(64) void k_narrow_thread(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
-                                                      const int* __restrict__ bounds, double dt) {
-  st_narrow_thread(worlds, q, bounds[NP_WAVE], (R)dt);
+template <typename R, int CLS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_narrow_cls(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
+                                                   const int* __restrict__ bounds, double dt) {
+  st_narrow_class<R, CLS>(worlds, q, bounds, (R)dt);
 }
 template <typename R>
 __global__ __launch_bounds__(256) void k_narrow_wave(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
@@ -965,8 +965,16 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls, ctx->d_q);
     if (ctx->big_nw)
       k_big_nplace<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls, ctx->d_q);
-    const int threads = std::max(64, std::min(qcap, 256 * 4 * 64));
-    k_narrow_thread<R><<<(threads + 63) / 64, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    // one kernel per thread class (each compiles only its resolver), up to 4 waves per SIMD
+    const int blocks = (std::max(64, std::min(qcap, 256 * 4 * 4 * 64)) + 63) / 64;
+    k_narrow_cls<R, NP_SS><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_cls<R, NP_SP><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_cls<R, NP_SB><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_cls<R, NP_SC><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_cls<R, NP_SH><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_cls<R, NP_PC><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_cls<R, NP_CC><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_cls<R, NP_OTHER><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
     const int waves = std::max(1, std::min(qcap, 4096));
     k_narrow_wave<R><<<(waves + 3) / 4, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
   }

@@ -51,9 +51,14 @@ template <typename R, bool WAVE = false> struct Sink {
   }
 };
 
+// resolver kinds a context compiles: the batched narrowphase runs each pair class through a
+// context with only that class's resolver, so its kernel holds the registers that resolver needs
+enum : unsigned { NK_SS = 1, NK_SP = 2, NK_SB = 4, NK_SC = 8, NK_SH = 16, NK_PC = 32, NK_CC = 64, NK_HC = 128,
+                  NK_ALL = 255 };
+
 // A resolver context.  WAVE: a whole wave runs one pair, every lane on the same data (so every
 // branch is uniform), and the separating-axis search splits its axes over the lanes.
-template <typename R, bool WAVE = false> struct NPCtx {
+template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   const DevWorld<R>& W;
   R dt;
   int cm;                 // currentContactMaterial
@@ -633,19 +638,22 @@ template <typename R, bool WAVE = false> struct NPCtx {
   CHP_HD void dispatch(int A, int B, f3 xa, f3 xb, f4 qa, f4 qb, int ba, int bb, SRef rsi, SRef rsj) {
     int ta = W.si_type[A], tb = W.si_type[B];
     SRef ra{W.si_response[A], W.si_material[A]}, rb{W.si_response[B], W.si_material[B]};
-    if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_SPHERE) sphereSphere(W.si_radius[A], W.si_radius[B], xa, xb, ba, bb, ra, rb, rsi, rsj);
-    else if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_PLANE) spherePlane(W.si_radius[A], xa, xb, qb, ba, bb, ra, rb, rsi, rsj);
-    else if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_BOX)
+    const bool convA = ta == CHP_SHAPE_BOX || ta == CHP_SHAPE_CONVEX;
+    if ((KINDS & NK_SS) && ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_SPHERE)
+      sphereSphere(W.si_radius[A], W.si_radius[B], xa, xb, ba, bb, ra, rb, rsi, rsj);
+    else if ((KINDS & NK_SP) && ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_PLANE)
+      spherePlane(W.si_radius[A], xa, xb, qb, ba, bb, ra, rb, rsi, rsj);
+    else if ((KINDS & NK_SB) && ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_BOX)
       sphereBox(W.si_radius[A], ((const f3*)W.si_half)[B], xa, xb, qb, ba, bb, ra, rb, rsi, rsj);
-    else if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_CONVEX)
+    else if ((KINDS & NK_SC) && ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_CONVEX)
       sphereConvex(W.si_radius[A], hull_view(W, W.si_hull[B]), xa, xb, qb, ba, bb, ra, rb, rsi, rsj);
-    else if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_HEIGHTFIELD)
+    else if ((KINDS & NK_SH) && ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_HEIGHTFIELD)
       sphereHeightfield(W.si_radius[A], W.si_bsr[A], W.si_hf[B], xa, xb, qb, ba, bb, ra, rb);
-    else if (ta == CHP_SHAPE_PLANE && (tb == CHP_SHAPE_BOX || tb == CHP_SHAPE_CONVEX))
+    else if ((KINDS & NK_PC) && ta == CHP_SHAPE_PLANE && (tb == CHP_SHAPE_BOX || tb == CHP_SHAPE_CONVEX))
       planeConvex(hull_view(W, W.si_hull[B]), xa, xb, qa, qb, ba, bb, ra, rb, rsi, rsj);
-    else if ((ta == CHP_SHAPE_BOX || ta == CHP_SHAPE_CONVEX) && (tb == CHP_SHAPE_BOX || tb == CHP_SHAPE_CONVEX))
+    else if ((KINDS & NK_CC) && convA && (tb == CHP_SHAPE_BOX || tb == CHP_SHAPE_CONVEX))
       convexConvex(hull_view(W, W.si_hull[A]), ra, hull_view(W, W.si_hull[B]), rb, xa, xb, qa, qb, ba, bb, rsi, rsj, false);
-    else if ((ta == CHP_SHAPE_BOX || ta == CHP_SHAPE_CONVEX) && tb == CHP_SHAPE_HEIGHTFIELD)
+    else if ((KINDS & NK_HC) && convA && tb == CHP_SHAPE_HEIGHTFIELD)
       heightfieldConvex(hull_view(W, W.si_hull[A]), ra, W.si_hf[B], xa, xb, qa, qb, ba, bb);
     // other combinations have no resolver in the covered set (narrow_phase.dart:699-700 skips)
   }

@@ -32,23 +32,10 @@ __device__ __forceinline__ int block_exscan(int v, int* sh, int* total) {
   __syncthreads();
   return incl - v;
 }
-// in-place exclusive scan of data[0..n); returns the total (same value in every thread)
+// in-place exclusive scan of data[0..n); returns the total (same value in every thread).  Each
+// thread scans a contiguous chunk: two passes over the data, one block-wide scan of the chunk
+// sums.
 __device__ __forceinline__ int array_exscan(int* data, int n, int* sh) {
-  int carry = 0;
-  for (int base = 0; base < n; base += blockDim.x) {
-    int i = base + threadIdx.x;
-    int v = i < n ? data[i] : 0;
-    int tot;
-    int ex = block_exscan(v, sh, &tot);
-    if (i < n) data[i] = carry + ex;
-    carry += tot;
-  }
-  __syncthreads();
-  return carry;
-}
-// the same for long arrays: each thread scans a contiguous chunk (two passes over the data, one
-// block-wide scan of the chunk sums)
-__device__ __forceinline__ int array_exscan_chunked(int* data, int n, int* sh) {
   const int c = (n + (int)blockDim.x - 1) / (int)blockDim.x;
   const int b = min(n, (int)threadIdx.x * c), e = min(n, b + c);
   int s = 0;
@@ -518,14 +505,17 @@ template <typename R> __device__ __forceinline__ void pair_counts(const DevWorld
   W.pc_off[p] = nc;
 }
 
-// one pair on one thread: contacts into the staging slab, per-pair counts
-template <typename R> __device__ __forceinline__ void narrow_pair(const DevWorld<R>& W, int p, R dt) {
+// one pair on one thread: contacts into the staging slab, per-pair counts.  KINDS: the
+// resolvers compiled in; KMAX: contacts buffered locally (a pair with more runs again straight
+// into the slab)
+template <typename R, unsigned KINDS = NK_ALL, int KMAX = CHP_KMAX>
+__device__ __forceinline__ void narrow_pair(const DevWorld<R>& W, int p, R dt) {
   int bi = W.pair_i[p], bj = W.pair_j[p];
   int n = 0, base = 0;
   if (bi >= 0) {
-    ContactRec<R> loc[CHP_KMAX];
-    Sink<R> s{loc, CHP_KMAX, 0, 0};
-    NPCtx<R> ctx(W, dt, &s);
+    ContactRec<R> loc[KMAX];
+    Sink<R> s{loc, KMAX, 0, 0};
+    NPCtx<R, false, KINDS> ctx(W, dt, &s);
     ctx.pair(bi, bj);
     if (ctx.status) ws_fail(W.counters, ctx.status);
     n = s.n;
@@ -534,11 +524,11 @@ template <typename R> __device__ __forceinline__ void narrow_pair(const DevWorld
       if (base + n > W.contact_cap) {
         ws_fail(W.counters, WS_CONTACTS);
         n = 0;
-      } else if (n <= CHP_KMAX) {
+      } else if (n <= KMAX) {
         for (int k = 0; k < n; k++) W.contacts[base + k] = loc[k];
       } else {  // more contacts than the local buffer: run the pair again into the slab
         Sink<R> s2{W.contacts + base, n, 0, 0};
-        NPCtx<R> ctx2(W, dt, &s2);
+        NPCtx<R, false, KINDS> ctx2(W, dt, &s2);
         ctx2.pair(bi, bj);
       }
     }
@@ -606,26 +596,41 @@ template <typename R> __device__ void st_narrow_place(const DevWorld<R>& W, int 
 // between the passes: offsets class-major over the listed worlds (cnt[w * NP_NCLS + c] becomes
 // the offset; bounds[c] = first entry of class c, bounds[NP_NCLS] = total).  One workgroup.
 __device__ inline void st_narrow_scan(int* cnt, int n, int* bounds, int* sh) {
+  // each thread owns a contiguous chunk of worlds; one block-wide scan per class
+  const int ch = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int w0 = min(n, (int)threadIdx.x * ch), w1 = min(n, w0 + ch);
   int carry = 0;
   for (int c = 0; c < NP_NCLS; c++) {
-    if (threadIdx.x == 0) bounds[c] = carry;
-    for (int base = 0; base < n; base += blockDim.x) {
-      int w = base + threadIdx.x;
-      int v = w < n ? cnt[w * NP_NCLS + c] : 0, tot;
-      int ex = block_exscan(v, sh, &tot);
-      if (w < n) cnt[w * NP_NCLS + c] = carry + ex;
-      carry += tot;
+    int s = 0, tot;
+    for (int w = w0; w < w1; w++) s += cnt[w * NP_NCLS + c];
+    int ex = carry + block_exscan(s, sh, &tot);
+    for (int w = w0; w < w1; w++) {
+      int v = cnt[w * NP_NCLS + c];
+      cnt[w * NP_NCLS + c] = ex;
+      ex += v;
     }
+    if (threadIdx.x == 0) bounds[c] = carry;
+    carry += tot;
   }
   if (threadIdx.x == 0) bounds[NP_NCLS] = carry;
 }
 
-// the thread queue, one pair per thread (grid-stride)
-template <typename R> __device__ void st_narrow_thread(const DevWorld<R>* worlds, const int2* q, int cnt, R dt) {
+// the resolvers of a thread class and the contacts its pairs make at most (sphere classes: one;
+// a pair that makes more still comes out right, run a second time into the slab)
+constexpr unsigned cls_kinds(int c) {
+  return c == NP_SS ? NK_SS : c == NP_SP ? NK_SP : c == NP_SB ? NK_SB : c == NP_SC ? NK_SC :
+         c == NP_SH ? NK_SH : c == NP_PC ? NK_PC : c == NP_CC ? NK_CC : NK_ALL;
+}
+constexpr int cls_kmax(int c) { return c <= NP_SC ? 1 : CHP_KMAX; }
+
+// one thread class of the queue, one pair per thread (grid-stride)
+template <typename R, int CLS>
+__device__ void st_narrow_class(const DevWorld<R>* worlds, const int2* q, const int* bounds, R dt) {
+  const int b1 = bounds[CLS + 1];
   const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x), ng = (int)(gridDim.x * blockDim.x);
-  for (int i = g; i < cnt; i += ng) {
+  for (int i = bounds[CLS] + g; i < b1; i += ng) {
     const int2 e = q[i];
-    narrow_pair(worlds[e.x], e.y, dt);
+    narrow_pair<R, cls_kinds(CLS), cls_kmax(CLS)>(worlds[e.x], e.y, dt);
   }
 }
 
