@@ -647,7 +647,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   std::vector<R> hbsr(H.H.bsr.begin(), H.H.bsr.end());
   ARR(h_bsr, hbsr);
   ARR(h_verts, H.H.verts); ARR(h_fnormal, H.H.fnormal); ARR(h_edges, H.H.edges);
-  ARR(f_istart, H.H.f_istart); ARR(f_idx, H.H.f_idx);
+  ARR(f_istart, H.H.f_istart); ARR(f_idx, H.H.f_idx); ARR(f_cstart, H.H.f_cstart); ARR(f_conn, H.H.f_conn);
   ARR(hf_nx, H.hf_nx); ARR(hf_ny, H.hf_ny); ARR(hf_dstart, H.hf_dstart); ARR(hf_pillar_hull, H.hf_pillar_hull);
   ARR(hf_min, H.hf_min); ARR(hf_max, H.hf_max); ARR(hf_es, H.hf_es); ARR(hf_data, H.hf_data);
   ARR(hf_pillar_off, H.hf_pillar_off);

@@ -24,6 +24,9 @@ struct HullBuf {  // hull table under construction
   std::vector<double> bsr;
   std::vector<f3> verts, fnormal, edges;
   std::vector<int> f_istart{0}, f_idx;
+  // per face: the hull's other faces that share a vertex with it, in face order (the connected
+  // faces clipFaceAgainstHull gathers for a witness face, convex_polyhedron.dart:448-462)
+  std::vector<int> f_cstart{0}, f_conn;
 };
 
 // ConvexPolyhedron.computeNormals / computeEdges / updateBoundingSphereRadius
@@ -47,6 +50,17 @@ int hull_add(HullBuf& H, const std::vector<f3>& verts, const std::vector<std::ve
     f3 n = cross3<R>(cb, ab);
     if (!is_zero3(n)) n = normalize3<R>(n);
     H.fnormal.push_back(neg3(n));
+  }
+  for (size_t w = 0; w < faces.size(); w++) {
+    for (size_t f = 0; f < faces.size(); f++) {
+      if (f == w) continue;
+      bool share = false;
+      for (int t : faces[f])
+        for (int u : faces[w])
+          if (t == u) share = true;
+      if (share) H.f_conn.push_back((int)f);
+    }
+    H.f_cstart.push_back((int)H.f_conn.size());
   }
   H.estart.push_back((int)H.edges.size());
   int e0 = (int)H.edges.size();

@@ -13,8 +13,14 @@ namespace chp {
 template <typename R> struct HullView {
   const f3* v; int nv;
   const f3* fn; const int* fidx; const int* fis; int nf;  // face f: fidx[fis[f] .. fis[f+1])
+  const int* fcs; const int* fconn;                       // face f's connected faces: fconn[fcs[f] .. fcs[f+1])
   const f3* e; int ne;
   bool axes; R bsr;
+};
+// the vertices of a hull of at most 8 (a box) held in registers for the separating-axis search
+struct HullVerts8 {
+  f3 v[8];
+  int nv;
 };
 
 template <typename R> CHP_HD HullView<R> hull_view(const DevWorld<R>& W, int h) {
@@ -24,6 +30,8 @@ template <typename R> CHP_HD HullView<R> hull_view(const DevWorld<R>& W, int h) 
   H.fn = (const f3*)W.h_fnormal + W.h_fstart[h];
   H.fis = W.f_istart + W.h_fstart[h];
   H.fidx = W.f_idx;
+  H.fcs = W.f_cstart + W.h_fstart[h];
+  H.fconn = W.f_conn;
   H.nf = W.h_fcount[h];
   H.e = (const f3*)W.h_edges + W.h_estart[h];
   H.ne = W.h_ecount[h];
@@ -335,9 +343,34 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     if (b > a) { R t = b; b = a; a = t; }
     mx = a; mn = b;
   }
+  // the same projection over register-held vertices (same order, same arithmetic)
+  CHP_HD static void project(const HullVerts8& H, f3 axis, f3 lo, f4 q, R& mx, R& mn) {
+    f3 la = vector_to_local<R>(q, axis);
+    R add = dot3<R>(lo, la);
+    R a = dot3<R>(H.v[0], la), b = a;
+#pragma unroll
+    for (int i = 1; i < 8; i++) {
+      if (i < H.nv) {
+        R val = dot3<R>(H.v[i], la);
+        if (val > a) a = val;
+        if (val < b) b = val;
+      }
+    }
+    b -= add;
+    a -= add;
+    if (b > a) { R t = b; b = a; a = t; }
+    mx = a; mn = b;
+  }
+  CHP_HD static HullVerts8 verts8(const HullView<R>& H) {
+    HullVerts8 c;
+    c.nv = H.nv;
+#pragma unroll
+    for (int i = 0; i < 8; i++) c.v[i] = i < H.nv ? H.v[i] : mk3(0.f, 0.f, 0.f);
+    return c;
+  }
   // testSepAxis (:360-384): false when separated
-  CHP_HD static bool testSepAxis(const HullView<R>& A, f3 axis, const HullView<R>& B, f3 loA, f4 qa, f3 loB, f4 qb,
-                                 R& depth) {
+  template <class VS>
+  CHP_HD static bool testSepAxis(const VS& A, f3 axis, const VS& B, f3 loA, f4 qa, f3 loB, f4 qb, R& depth) {
     R maxA, minA, maxB, minB;
     project(A, axis, loA, qa, maxA, minA);
     project(B, axis, loB, qb, maxB, minB);
@@ -346,10 +379,23 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     depth = d0 < d1 ? d0 : d1;
     return true;
   }
-  // findSeparatingAxis (:232-356); face0_only: faceListA = [0]
+  // findSeparatingAxis (:232-356); face0_only: faceListA = [0].  Hulls of at most 8 vertices are
+  // projected from registers.
   CHP_HD static bool findSepAxis(const HullView<R>& A, const HullView<R>& B, f3 pa, f4 qa, f3 pb, f4 qb, bool face0_only,
                                  f3& target) {
-    if constexpr (WAVE) return findSepAxisWave(A, B, pa, qa, pb, qb, face0_only, target);
+    if constexpr (WAVE) {
+      return findSepAxisWave(A, B, pa, qa, pb, qb, face0_only, target);
+    } else {
+      if (A.nv <= 8 && B.nv <= 8) {
+        const HullVerts8 va = verts8(A), vb = verts8(B);
+        return findSepAxisT(A, va, B, vb, pa, qa, pb, qb, face0_only, target);
+      }
+      return findSepAxisT(A, A, B, B, pa, qa, pb, qb, face0_only, target);
+    }
+  }
+  template <class VS>
+  CHP_HD static bool findSepAxisT(const HullView<R>& A, const VS& vA, const HullView<R>& B, const VS& vB, f3 pa, f4 qa,
+                                  f3 pb, f4 qb, bool face0_only, f3& target) {
     R dmin = (R)__builtin_huge_val();
     const f3 zero = mk3(0.f, 0.f, 0.f);
     f3 loA = point_to_local<R>(pa, qa, zero), loB = point_to_local<R>(pb, qb, zero);
@@ -358,7 +404,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
       for (int i = 0; i < nA; i++) {
         f3 n = qrot<R>(qa, A.fn[i]);
         R d;
-        if (!testSepAxis(A, n, B, loA, qa, loB, qb, d)) return false;
+        if (!testSepAxis(vA, n, vB, loA, qa, loB, qb, d)) return false;
         if (d < dmin) { dmin = d; target = n; }
       }
     }
@@ -366,7 +412,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
       for (int i = 0; i < B.nf; i++) {
         f3 n = qrot<R>(qb, B.fn[i]);
         R d;
-        if (!testSepAxis(A, n, B, loA, qa, loB, qb, d)) return false;
+        if (!testSepAxis(vA, n, vB, loA, qa, loB, qb, d)) return false;
         if (d < dmin) { dmin = d; target = n; }
       }
     }
@@ -378,7 +424,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
         if (!almost_zero3<R>(cr)) {
           cr = normalize3<R>(cr);
           R d;
-          if (!testSepAxis(A, cr, B, loA, qa, loB, qb, d)) return false;
+          if (!testSepAxis(vA, cr, vB, loA, qa, loB, qb, d)) return false;
           if (d < dmin) { dmin = d; target = cr; }
         }
       }
@@ -489,21 +535,13 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     }
     if (cfa < 0) return;
     int pa0 = A.fis[cfa], nva = A.fis[cfa + 1] - pa0;
-    // connected faces: faces sharing a vertex with the witness face, in face order
-    int conn[CHP_MAXPOLY];
-    int nconn = 0;
-    for (int f = 0; f < A.nf; f++) {
-      if (f == cfa) continue;
-      bool share = false;
-      for (int t = A.fis[f]; t < A.fis[f + 1] && !share; t++)
-        for (int u = 0; u < nva; u++)
-          if (A.fidx[pa0 + u] == A.fidx[t]) { share = true; break; }
-      if (share) { if (nconn < CHP_MAXPOLY) conn[nconn] = f; nconn++; }
-    }
+    // connected faces: faces sharing a vertex with the witness face, in face order (precomputed
+    // per face with the hull)
+    const int c0 = A.fcs[cfa], nconn = A.fcs[cfa + 1] - c0;
     f3* pin = bufA;
     f3* pout = bufB;
     for (int i = 0; i < nva; i++) {
-      int other = (nconn > 0 && nconn > i) ? (i < CHP_MAXPOLY ? conn[i] : 0) : 0;
+      int other = (nconn > 0 && nconn > i) ? A.fconn[c0 + i] : 0;
       f3 lpn = A.fn[other];
       R lpe = -dot3<R>(lpn, A.v[A.fidx[A.fis[other]]]);
       f3 pnws = qrot<R>(qi, lpn);

@@ -78,6 +78,7 @@ template <typename R> struct DevWorld {
   CHP_G(R) h_bsr;
   CHP_G(float) h_verts, h_fnormal, h_edges;
   CHP_G(int) f_istart, f_idx;
+  CHP_G(int) f_cstart, f_conn;   // connected faces of each face (f_cstart indexed like f_istart)
   // heightfields
   int nhf, pad2;
   CHP_G(int) hf_nx, hf_ny, hf_dstart, hf_pillar_hull;
