@@ -305,6 +305,7 @@ def main():
                  "sample": "%d worlds (every %d-th), 5 steps each, one launch per stage per step "
                            "and a host synchronisation per step" % (len(sample), stride)}
 
+    launch_shape = ctx.tuning()
     serving = None
     if not args.no_serving:
         ctx.close()
@@ -330,7 +331,7 @@ def main():
         "dtype": args.precision,
         "data": "synthetic",
         "config": {"workload": desc, "worlds": len(recs), "dynamic_bodies": n_dyn, "dt": dt,
-                   "launch_shape": ctx.tuning(),
+                   "launch_shape": launch_shape,
                    "pairs_per_step": P, "contacts_per_step": Cn, "rows_per_step": E},
         "roofline": {"bound": "hbm", "kernel": "k_gs", "achieved": achieved_gs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved_gs / HBM_PEAK_GBS,
