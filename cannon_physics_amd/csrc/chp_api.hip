@@ -243,7 +243,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 template <typename R>
 __global__ __launch_bounds__(256) void k_narrow_wave(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
                                                      const int* __restrict__ bounds, double dt) {
-  st_narrow_wave(worlds, q + bounds[NP_WAVE], bounds[NP_NCLS] - bounds[NP_WAVE], (R)dt);
+  __shared__ f3 ws[4][NPCtx<R, true>::WS_F3];   // one LDS workspace per wave
+  st_narrow_wave(worlds, q + bounds[NP_WAVE], bounds[NP_NCLS] - bounds[NP_WAVE], (R)dt, ws[threadIdx.x >> 6]);
 }
 
 

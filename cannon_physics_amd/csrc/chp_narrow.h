@@ -72,7 +72,15 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   int cm;                 // currentContactMaterial
   Sink<R, WAVE>* sink;
   int status;
-  CHP_HD NPCtx(const DevWorld<R>& w, R dt_, Sink<R, WAVE>* s) : W(w), dt(dt_), cm(0), sink(s), status(0) {}
+  // WAVE: the wave's LDS workspace.  Every lane holds the same pair, so one copy per wave
+  // serves them all: the two hulls' vertices and world-space edge directions of the current
+  // convexConvex (hulls up to WS_NV vertices / WS_NE edges) and the two clip polygons.
+  f3* ws = nullptr;
+  static constexpr int WS_NV = 32, WS_NE = 64;
+  static constexpr int WS_VA = 0, WS_VB = WS_NV, WS_EA = 2 * WS_NV, WS_EB = WS_EA + WS_NE, WS_CA = WS_EB + WS_NE,
+                       WS_CB = WS_CA + CHP_MAXPOLY, WS_F3 = WS_CB + CHP_MAXPOLY;   // offsets in f3
+  CHP_HD NPCtx(const DevWorld<R>& w, R dt_, Sink<R, WAVE>* s, f3* ws_ = nullptr)
+      : W(w), dt(dt_), cm(0), sink(s), status(0), ws(ws_) {}
 
   // Narrowphase.createContactEquation (narrow_phase.dart:492-528) +
   // createFrictionEquationsFromContact (:530-586) + the World's body-material restitution
@@ -333,6 +341,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     f3 la = vector_to_local<R>(q, axis);
     R add = dot3<R>(lo, la);
     R a = dot3<R>(H.v[0], la), b = a;
+#pragma unroll 4
     for (int i = 1; i < H.nv; i++) {
       R val = dot3<R>(H.v[i], la);
       if (val > a) a = val;
@@ -438,7 +447,8 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   // first axis of least depth (strict <): here any lane's separating axis answers the first, a
   // (depth, axis index) minimum over the lanes the second.
   __device__ static bool findSepAxisWave(const HullView<R>& A, const HullView<R>& B, f3 pa, f4 qa, f3 pb, f4 qb,
-                                         bool face0_only, f3& target) {
+                                         bool face0_only, f3& target, const f3* weA = nullptr,
+                                         const f3* weB = nullptr) {
     const int lane = threadIdx.x & 63;
     const f3 zero = mk3(0.f, 0.f, 0.f);
     f3 loA = point_to_local<R>(pa, qa, zero), loB = point_to_local<R>(pb, qb, zero);
@@ -448,26 +458,31 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     int bestk = 0x7fffffff;
     f3 bax = zero;
     bool sep = false;
-    for (int k = lane; k < total; k += 64) {
-      f3 axis;
-      bool valid = true;
-      if (k < nA) {
-        axis = qrot<R>(qa, A.fn[k]);
-      } else if (k < nA + nB) {
-        axis = qrot<R>(qb, B.fn[k - nA]);
-      } else {
-        int e = k - nA - nB, e0 = e / B.ne, e1 = e - e0 * B.ne;
-        f3 cr = cross3<R>(qrot<R>(qa, A.e[e0]), qrot<R>(qb, B.e[e1]));
-        valid = !almost_zero3<R>(cr);
-        axis = valid ? normalize3<R>(cr) : zero;
+    // 64 axes per round; a separating axis in any round answers "separated" at once
+    for (int k0 = 0; k0 < total; k0 += 64) {
+      const int k = k0 + lane;
+      if (k < total) {
+        f3 axis;
+        bool valid = true;
+        if (k < nA) {
+          axis = qrot<R>(qa, A.fn[k]);
+        } else if (k < nA + nB) {
+          axis = qrot<R>(qb, B.fn[k - nA]);
+        } else {
+          int e = k - nA - nB, e0 = e / B.ne, e1 = e - e0 * B.ne;
+          const f3 ea = weA ? weA[e0] : qrot<R>(qa, A.e[e0]), eb = weB ? weB[e1] : qrot<R>(qb, B.e[e1]);
+          f3 cr = cross3<R>(ea, eb);
+          valid = !almost_zero3<R>(cr);
+          axis = valid ? normalize3<R>(cr) : zero;
+        }
+        if (valid) {
+          R d;
+          if (!testSepAxis(A, axis, B, loA, qa, loB, qb, d)) sep = true;
+          else if (d < best) { best = d; bestk = k; bax = axis; }
+        }
       }
-      if (valid) {
-        R d;
-        if (!testSepAxis(A, axis, B, loA, qa, loB, qb, d)) sep = true;
-        else if (d < best) { best = d; bestk = k; bax = axis; }
-      }
+      if (__any(sep)) return false;
     }
-    if (__any(sep)) return false;
     // lexicographic (depth, axis index) minimum over the wave
     int win = lane;
     for (int off = 32; off > 0; off >>= 1) {
@@ -509,11 +524,41 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   }
   // convexConvex (narrow_phase.dart:1981-2043) with clipAgainstHull / clipFaceAgainstHull
   // (convex_polyhedron.dart:189-227, 417-541; their unused temporaries are omitted)
-  CHP_HD void convexConvex(const HullView<R>& A, SRef siRef, const HullView<R>& B, SRef sjRef, f3 xi, f3 xj, f4 qi,
+  // WAVE: lanes copy n items (rotated by q when rot) into the workspace; then the wave sees them
+  __device__ static void stage(f3* dst, const f3* src, int n, bool rot, f4 q) {
+    for (int i = (int)(threadIdx.x & 63); i < n; i += 64) dst[i] = rot ? qrot<R>(q, src[i]) : src[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  CHP_HD void convexConvex(const HullView<R>& A0, SRef siRef, const HullView<R>& B0, SRef sjRef, f3 xi, f3 xj, f4 qi,
                            f4 qj, int bi, int bj, SRef rsi, SRef rsj, bool face0_only) {
-    if (dist3<R>(xi, xj) > A.bsr + B.bsr) return;
+    if (dist3<R>(xi, xj) > A0.bsr + B0.bsr) return;
+    HullView<R> A = A0, B = B0;
     f3 sep = mk3(0.f, 0.f, 0.f);
-    if (!findSepAxis(A, B, xi, qi, xj, qj, face0_only, sep)) return;
+    f3 lbuf[WAVE ? 1 : 2 * CHP_MAXPOLY];
+    f3* bufA = lbuf;
+    f3* bufB = lbuf + (WAVE ? 0 : CHP_MAXPOLY);
+    if constexpr (WAVE) {
+      // vertices and world edge directions from the workspace (the same values the global
+      // tables give), clip polygons in it
+      const f3 *weA = nullptr, *weB = nullptr;
+      if (A.nv <= WS_NV && B.nv <= WS_NV && A.ne <= WS_NE && B.ne <= WS_NE) {
+        stage(ws + WS_VA, A0.v, A.nv, false, qi);
+        stage(ws + WS_VB, B0.v, B.nv, false, qj);
+        stage(ws + WS_EA, A0.e, A.ne, true, qi);
+        stage(ws + WS_EB, B0.e, B.ne, true, qj);
+        A.v = ws + WS_VA;
+        B.v = ws + WS_VB;
+        weA = ws + WS_EA;
+        weB = ws + WS_EB;
+      }
+      bufA = ws + WS_CA;
+      bufB = ws + WS_CB;
+      if (!findSepAxisWave(A, B, xi, qi, xj, qj, face0_only, sep, weA, weB)) return;
+    } else {
+      if (!findSepAxis(A, B, xi, qi, xj, qj, face0_only, sep)) return;
+    }
     // clipAgainstHull: B's face most aligned with the axis
     int cfb = -1;
     R dmax = -(R)__builtin_huge_val();
@@ -522,7 +567,6 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
       if (d > dmax) { dmax = d; cfb = f; }
     }
     if (cfb < 0) return;
-    f3 bufA[CHP_MAXPOLY], bufB[CHP_MAXPOLY];
     int nin = B.fis[cfb + 1] - B.fis[cfb];
     if (nin > CHP_MAXPOLY) { status |= WS_STAGING; return; }
     for (int k = 0; k < nin; k++) bufA[k] = add3<R>(qrot<R>(qj, B.v[B.fidx[B.fis[cfb] + k]]), xj);

@@ -635,7 +635,8 @@ __device__ void st_narrow_class(const DevWorld<R>* worlds, const int2* q, const 
 }
 
 // the queued separating-axis pairs, one wave per pair (grid-stride over the queue)
-template <typename R> __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, R dt) {
+template <typename R>
+__device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, R dt, f3* ws) {
   const int lane = threadIdx.x & 63;
   const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
   for (int i = gw; i < cnt; i += nw) {
@@ -645,7 +646,7 @@ template <typename R> __device__ void st_narrow_wave(const DevWorld<R>* worlds, 
     const int bi = W.pair_i[p], bj = W.pair_j[p];
     ContactRec<R> loc[CHP_KMAX];
     Sink<R, true> s{loc, CHP_KMAX, 0, 0};
-    NPCtx<R, true> ctx(W, dt, &s);
+    NPCtx<R, true> ctx(W, dt, &s, ws);
     ctx.pair(bi, bj);
     int n = s.n, base = 0;
     if (lane == 0) {
@@ -658,7 +659,7 @@ template <typename R> __device__ void st_narrow_wave(const DevWorld<R>* worlds, 
       n = 0;
     } else if (n > CHP_KMAX) {  // more contacts than the local buffer: run the pair again into the slab
       Sink<R, true> s2{W.contacts + base, n, 0, 0};
-      NPCtx<R, true> ctx2(W, dt, &s2);
+      NPCtx<R, true> ctx2(W, dt, &s2, ws);
       ctx2.pair(bi, bj);
     } else if (lane == 0) {
       for (int k = 0; k < n; k++) W.contacts[base + k] = loc[k];
