@@ -76,6 +76,8 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   // serves them all: the two hulls' vertices and world-space edge directions of the current
   // convexConvex (hulls up to WS_NV vertices / WS_NE edges) and the two clip polygons.
   f3* ws = nullptr;
+  const f3* ws_a = nullptr;   // the hull (vertex table) and orientation whose data WS_VA / WS_EA hold
+  f4 ws_qa{};
   static constexpr int WS_NV = 32, WS_NE = 64;
   static constexpr int WS_VA = 0, WS_VB = WS_NV, WS_EA = 2 * WS_NV, WS_EB = WS_EA + WS_NE, WS_CA = WS_EB + WS_NE,
                        WS_CB = WS_CA + CHP_MAXPOLY, WS_F3 = WS_CB + CHP_MAXPOLY;   // offsets in f3
@@ -544,9 +546,18 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
       // tables give), clip polygons in it
       const f3 *weA = nullptr, *weB = nullptr;
       if (A.nv <= WS_NV && B.nv <= WS_NV && A.ne <= WS_NE && B.ne <= WS_NE) {
-        stage(ws + WS_VA, A0.v, A.nv, false, qi);
+        // A is the same hull at the same pose for every pillar of a heightfield pair: staged once
+        const bool same = ws_a == A0.v && __float_as_uint(ws_qa.x) == __float_as_uint(qi.x) &&
+                          __float_as_uint(ws_qa.y) == __float_as_uint(qi.y) &&
+                          __float_as_uint(ws_qa.z) == __float_as_uint(qi.z) &&
+                          __float_as_uint(ws_qa.w) == __float_as_uint(qi.w);
+        if (!same) {
+          stage(ws + WS_VA, A0.v, A.nv, false, qi);
+          stage(ws + WS_EA, A0.e, A.ne, true, qi);
+          ws_a = A0.v;
+          ws_qa = qi;
+        }
         stage(ws + WS_VB, B0.v, B.nv, false, qj);
-        stage(ws + WS_EA, A0.e, A.ne, true, qi);
         stage(ws + WS_EB, B0.e, B.ne, true, qj);
         A.v = ws + WS_VA;
         B.v = ws + WS_VB;
