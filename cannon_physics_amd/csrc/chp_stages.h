@@ -324,7 +324,7 @@ template <typename R> __device__ void st_filter(const DevWorld<R>& W) {
 // SAP_TILE keys (bitonic sort in LDS, one workgroup per tile) that are then merged pairwise by
 // rank, and the pairs are counted, scanned and written in three grid-wide passes.  Sort order,
 // pair order and the collideConnected filter are those of the workgroup path above.
-constexpr int BP_BIG = 1024, BP_SPAN = 64, SAP_TILE = 4096;
+constexpr int BP_BIG = 512, BP_SPAN = 64, SAP_TILE = 4096;
 
 template <typename R> __device__ __forceinline__ bool bp_filtered(const DevWorld<R>& W, int a, int b) {
   for (int c = 0; c < W.ncons; c++) {

@@ -105,7 +105,7 @@ template <typename R> struct DevWorld {
   CHP_G(void) gsr;   // GS records in level order (24 R-sized words per row)
   CHP_G(float) vw;   // vlambda / wlambda, 8 floats per body, when the world is too big for LDS
   CHP_G(unsigned long long) sort_keys;
-  // SAP worlds of at least 1024 bodies: the sweep's body data in axis-list order (position and
+  // large SAP worlds (see BP_BIG): the sweep's body data in axis-list order (position and
   // body index, 4 words; filter group, mask, static flag, 4 words; bounding radius)
   CHP_G(float) sbp;
   CHP_G(int) sgm;

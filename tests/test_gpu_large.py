@@ -1,4 +1,4 @@
-"""Large worlds (at least 1024 bodies), whose broadphase runs grid-wide: the prestep and the pair
+"""Large worlds (at least 512 bodies), whose broadphase runs grid-wide: the prestep and the pair
 passes over spans of rows, the SAP axis list sorted in tiles and merged.  Same bar as the other
 parity tests: bit-identical pairs, rows and state against the oracle in the same arithmetic."""
 import numpy as np
