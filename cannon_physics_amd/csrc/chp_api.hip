@@ -516,6 +516,8 @@ struct chp_ctx {
   int* d_list = nullptr;
   size_t d_list_cap = 0;
   std::vector<int> h_list;  // host copy of d_list (the source of its last upload)
+  std::vector<int> prev_list;   // the list and dt of the previous chp_step call
+  double prev_dt = 0;
   // the large-world tables of the current list: spans, SAP tiles (int2 each), world list indices
   int* d_big = nullptr;
   size_t d_big_cap = 0;
@@ -1022,12 +1024,15 @@ static void drop_graph(chp_ctx* ctx) {
 // the captured step for this configuration (captured anew when anything baked into its kernel
 // arguments changed)
 template <typename R>
-static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int max_nb, double dt, hipGraphExec_t* out) {
+static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int max_nb, double dt, bool may_capture,
+                      hipGraphExec_t* out) {
   chp_ctx::StepGraph& g = ctx->sg;
+  *out = nullptr;
   if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block &&
         g.bg == ctx->solve_block && g.dq == ctx->d_q && g.dqcls == ctx->d_qcls && g.dlist == ctx->d_list &&
         g.dbig == ctx->d_big && g.big == std::array<int, 4>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds} &&
         g.dworlds == ctx->d_worlds)) {
+    if (!may_capture) return CHP_OK;
     drop_graph(ctx);
     HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     int r = launch_step<R>(ctx, ctx->d_list, (int)list.size(), nbig, max_nb, dt, nullptr);
@@ -1080,10 +1085,15 @@ template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, ui
     ctx->tev.push_back(e);
   }
   const int max_nb = ctx->worlds[list[0]].nb;
-  // steps after the first of a call replay a captured graph of the step (no per-step events)
+  // the steps of a call replay a captured graph of the step (no per-step events).  A capture
+  // costs about as much as a few eager steps, so a new configuration is captured only when the
+  // call is long or the previous call stepped the same list (a stable batch); else it runs eager.
   hipGraphExec_t exec = nullptr;
+  const bool stable = ctx->prev_list == list && ctx->prev_dt == dt;
+  ctx->prev_list = list;
+  ctx->prev_dt = dt;
   if (ctx->use_graphs && !ctx->profile && nsteps > 1) {
-    r = step_graph<R>(ctx, list, nbig, max_nb, dt, &exec);
+    r = step_graph<R>(ctx, list, nbig, max_nb, dt, stable || nsteps >= 16, &exec);
     if (r != CHP_OK) return r;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
