@@ -261,16 +261,23 @@ __global__ __launch_bounds__(128) void k_rows(const DevWorld<R>* __restrict__ wo
 // Levels and GS sweeps run one large world per workgroup (the first nbig entries of the list)
 // and one small world (at most SMALL_NB bodies) per wave, blockDim / 64 of them per workgroup.
 constexpr int SMALL_NB = 128;        // bodies of a world run by one wave
-constexpr int BIG_NB_LDS = 512;      // bodies of a large world whose GS / level state fits in LDS
-constexpr int LS_BIG = 1024, LS_SMALL = 256;   // level starts kept in LDS
+constexpr int LS_BIG = 2048, LS_SMALL = 1024;   // level starts kept in LDS
+// bodies of a large world whose GS / level state fits in LDS: at least 512, up to the largest
+// large world of the launch (the launch's dynamic LDS is sized for it), at most what a
+// workgroup's LDS holds beside the level starts and the static arrays (4 KiB kept free)
+constexpr int BIG_NB_LDS_MIN = 512;
+static int big_lds_cap(int nbig, int max_nb, size_t max_lds) {
+  const int most = (int)((max_lds - 4096 - (size_t)LS_BIG * sizeof(int)) / (8 * sizeof(float)));
+  return nbig ? std::max(BIG_NB_LDS_MIN, std::min(max_nb, most)) : BIG_NB_LDS_MIN;
+}
 
-// dynamic LDS of k_levels / k_gs for a workgroup of bd threads
-static size_t lds_levels(int bd) {
-  size_t big = (size_t)(BIG_NB_LDS + LS_BIG) * sizeof(int), small = (size_t)(SMALL_NB + LS_SMALL) * sizeof(int);
+// dynamic LDS of k_levels / k_gs for a workgroup of bd threads (big_cap: as big_lds_cap)
+static size_t lds_levels(int bd, int big_cap) {
+  size_t big = (size_t)(big_cap + LS_BIG) * sizeof(int), small = (size_t)(SMALL_NB + LS_SMALL) * sizeof(int);
   return std::max(big, (bd / 64) * small);
 }
-static size_t lds_gs(int bd) {
-  size_t big = (size_t)BIG_NB_LDS * 8 * sizeof(float) + LS_BIG * sizeof(int);
+static size_t lds_gs(int bd, int big_cap) {
+  size_t big = (size_t)big_cap * 8 * sizeof(float) + LS_BIG * sizeof(int);
   size_t small = (size_t)SMALL_NB * 8 * sizeof(float) + LS_SMALL * sizeof(int);
   return std::max(big, (bd / 64) * small);
 }
@@ -294,7 +301,7 @@ __device__ __forceinline__ int group_world(const DevWorld<R>* worlds, const int*
 // GS dependency levels of the step's rows and the level-ordered GS records
 template <typename R>
 __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                                int nbig, int n) {
+                                                int nbig, int n, int big_cap) {
   __shared__ int sh[256];
   extern __shared__ int lds_i[];
   int wave;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ 
   if (id < 0) return;
   const DevWorld<R>& W = worlds[id];
   if (wave < 0) {
-    if (W.nb <= BIG_NB_LDS) st_levels<R, GBlock, true>(W, GBlock(), sh, lds_i, lds_i + BIG_NB_LDS, LS_BIG);
+    if (W.nb <= big_cap) st_levels<R, GBlock, true>(W, GBlock(), sh, lds_i, lds_i + big_cap, LS_BIG);
     else st_levels<R, GBlock, false>(W, GBlock(), sh, W.tmp, nullptr, 0);
   } else {
     int* base = lds_i + wave * (SMALL_NB + LS_SMALL);
@@ -314,7 +321,7 @@ __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ 
 // GS iterations
 template <typename R>
 __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                            int nbig, int n) {
+                                            int nbig, int n, int big_cap) {
   __shared__ int sh[256];
   __shared__ double shd[256];
   extern __shared__ float4 lds_f4[];
@@ -325,8 +332,8 @@ __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worl
   float* lds = (float*)lds_f4;
   const int L = W.counters[C_LEVELS];
   if (wave < 0) {
-    if (W.nb <= BIG_NB_LDS && L + 1 <= LS_BIG)
-      st_gs<R, GBlock, true>(W, GBlock(), shd, sh, lds, (int*)(lds + 8 * BIG_NB_LDS));
+    if (W.nb <= big_cap && L + 1 <= LS_BIG)
+      st_gs<R, GBlock, true>(W, GBlock(), shd, sh, lds, (int*)(lds + 8 * big_cap));
     else
       st_gs<R, GBlock, false>(W, GBlock(), shd, sh, W.vw, nullptr);
   } else {
@@ -949,6 +956,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
                                              hipEvent_t* ev, bool tune = false) {
   const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
   auto groups = [&](int bd) { return nbig + (n - nbig + bd / 64 - 1) / (bd / 64); };
+  const int bcap = big_lds_cap(nbig, max_nb, ctx->max_lds);
   hipStream_t st = ctx->stream;
   if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
   {
@@ -992,9 +1000,9 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     for (int c = 0; c < 3; c++) {
       const int b = kTuneBlocks[c];
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c], st));
-      k_levels<R><<<groups(b), b, lds_levels(b), st>>>(dw, d_list, nbig, n);
+      k_levels<R><<<groups(b), b, lds_levels(b, bcap), st>>>(dw, d_list, nbig, n, bcap);
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 1], st));
-      k_gs<R><<<groups(b), b, lds_gs(b), st>>>(dw, d_list, nbig, n);
+      k_gs<R><<<groups(b), b, lds_gs(b, bcap), st>>>(dw, d_list, nbig, n, bcap);
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 2], st));
     }
     if (ev) {
@@ -1003,9 +1011,9 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     }
   } else {
     const int bl = ctx->levels_block, bg = ctx->solve_block;
-    k_levels<R><<<groups(bl), bl, lds_levels(bl), st>>>(dw, d_list, nbig, n);
+    k_levels<R><<<groups(bl), bl, lds_levels(bl, bcap), st>>>(dw, d_list, nbig, n, bcap);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
-    k_gs<R><<<groups(bg), bg, lds_gs(bg), st>>>(dw, d_list, nbig, n);
+    k_gs<R><<<groups(bg), bg, lds_gs(bg, bcap), st>>>(dw, d_list, nbig, n, bcap);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
   }
   if (ctx->big_nw) k_big_post<R><<<ctx->big_nspans, 64, 0, st>>>(dw, d_list, big_spans(ctx), dt);
@@ -1384,8 +1392,9 @@ static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t 
   const int bl = ctx->levels_block, bg = ctx->solve_block, nbig = S.nb > SMALL_NB ? 1 : 0;
   hipStream_t st = ctx->stream;
   k_solve_rows<R><<<1, 256, 0, st>>>(dw, (int)id, ctx->d_eq, (int)n, dt);
-  k_levels<R><<<1, bl, lds_levels(bl), st>>>(dw, ctx->d_list, nbig, 1);
-  k_gs<R><<<1, bg, lds_gs(bg), st>>>(dw, ctx->d_list, nbig, 1);
+  const int bcap = big_lds_cap(nbig, S.nb, ctx->max_lds);
+  k_levels<R><<<1, bl, lds_levels(bl, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap);
+  k_gs<R><<<1, bg, lds_gs(bg, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap);
   k_solve_finish<R><<<1, 256, 0, st>>>(dw, (int)id);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(st));
