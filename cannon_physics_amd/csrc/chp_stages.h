@@ -1314,53 +1314,77 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
   R* adl = W.absdl;
   if (E != 0 && iters > 0) {
     GsRow<R> q[D];
-    R ql[D];
+    R ql[D];      // lambda as loaded with the slot's row
+    R qf[D];      // lambda forwarded to the slot (its row ran again after the load)
+    bool qo[D];   // qf, not ql, holds the slot's lambda
     int qk[D];
     bool qv[D];
-    int pl = 0;   // level of the next step to load
+    // a step is one group-wide slice of a level: a level of c rows takes ceil(c / bd) steps
+    // (its rows are independent, so slicing it keeps the sequential sweep's values)
+    int pl = 0, pj = 0;   // level and slice of the next step to load
+    // the step's row of this thread (k, valid) and the next step's (level, slice)
+    auto next = [&](int& k, bool& valid) {
+      const int a0 = ls[pl], a1 = ls[pl + 1];
+      k = a0 + pj * bd + tid;
+      valid = k < a1;
+      if (a0 + (pj + 1) * bd < a1) {
+        pj++;
+      } else {
+        pj = 0;
+        pl = (pl + 1 == L) ? 0 : pl + 1;
+      }
+    };
+    // steps per sweep
+    int lp = 0;
+    for (int l = tid; l < L; l += bd) {
+      const int c = ls[l + 1] - ls[l];
+      lp += c > bd ? (c + bd - 1) / bd : 1;
+    }
+    const int Lp = uni((int)g.sum((double)lp, shd));
 #pragma unroll
     for (int s = 0; s < D; s++) {
-      int k = ls[pl] + tid;
-      qv[s] = k < ls[pl + 1];
+      int k;
+      next(k, qv[s]);
       qk[s] = qv[s] ? k : 0;
       gs_load<R>(gsr, qk[s], q[s]);
       ql[s] = lam[qk[s]];
-      pl = (pl + 1 == L) ? 0 : pl + 1;
+      qf[s] = (R)0;
+      qo[s] = false;
+      // keep the slots' loads in slot order, as the steady state issues them: the wait before
+      // the first step of the loop then allows the later slots to stay in flight
+      __builtin_amdgcn_sched_barrier(0);
     }
-    int l = 0;
+    int step = 0;   // of the sweep
     double part = 0.0;
     for (;;) {
 #pragma unroll
       for (int s = 0; s < D; s++) {
         const int k = qk[s];
         const bool v = qv[s];
-        const R lnew = gs_row<R>(q[s], ql[s], v, v ? k : dummy, vw, lam, adl, part);
+        const R lnew = gs_row<R>(q[s], qo[s] ? qf[s] : ql[s], v, v ? k : dummy, vw, lam, adl, part);
+        // forwarding touches only qf / qo: a select on ql would wait for loads still in flight
 #pragma unroll
         for (int s2 = 0; s2 < D; s2++)
-          if (s2 != s && v && qv[s2] && qk[s2] == k) ql[s2] = lnew;
+          if (s2 != s && v && qv[s2] && qk[s2] == k) {
+            qf[s2] = lnew;
+            qo[s2] = true;
+          }
         {  // the slot's next row (D steps on)
-          int kn = ls[pl] + tid;
-          bool vn = kn < ls[pl + 1];
+          int kn;
+          bool vn;
+          next(kn, vn);
           int kc = vn ? kn : 0;
           gs_load<R>(gsr, kc, q[s]);
-          R lm = lam[kc];
-          ql[s] = (v && vn && kc == k) ? lnew : lm;
+          ql[s] = lam[kc];
+          qf[s] = lnew;
+          qo[s] = v && vn && kc == k;
           qk[s] = kc;
           qv[s] = vn;
-          pl = (pl + 1 == L) ? 0 : pl + 1;
         }
-        // further rows of this thread in a level wider than the group
-        if (uni(ls[l + 1] - ls[l]) > bd) {
-          for (int k2 = ls[l] + tid + bd; k2 < ls[l + 1]; k2 += bd) {
-            GsRow<R> r;
-            gs_load<R>(gsr, k2, r);
-            (void)gs_row<R>(r, lam[k2], true, k2, vw, lam, adl, part);
-          }
-        }
-        if constexpr (LDS) g.lds_sync();   // the next level reads what this one wrote
+        if constexpr (LDS) g.lds_sync();   // the next step reads what this one wrote
         else g.sync();
-        if (++l == L) {   // end of a sweep: GSSolver's early-exit test
-          l = 0;
+        if (++step == Lp) {   // end of a sweep: GSSolver's early-exit test
+          step = 0;
           double Dsum = g.sum(part, shd);
           int dec = gs_decide<R>(Dsum, E, tol2);
           if (uni(dec) < 0) {
