@@ -187,15 +187,7 @@ __global__ __launch_bounds__(1024) void k_big_rowscan(const DevWorld<R>* __restr
     }
   }
 }
-template <typename R>
-__global__ __launch_bounds__(256) void k_big_rowbuild(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                                      const int2* __restrict__ spans, double dt) {
-  CHP_SPAN_PROLOGUE(spans)
-  const int K = (W.nb + BP_SPAN - 1) / BP_SPAN, k = sp.y / BP_SPAN, P = W.counters[C_PAIRS];
-  const int F = W.counters[C_FROWS], C = W.counters[C_CROWS];
-  for (int p = k * 256 + (int)threadIdx.x; p < P; p += K * 256) pair_rows(W, p, F, (R)dt);
-  for (int c = k * 256 + (int)threadIdx.x; c < W.ncons; c += K * 256) constraint_rows(W, c, F + C + W.c_row_off[c], (R)dt);
-}
+
 
 // integration of a large world's bodies over its spans; a non-finite pose is noted in
 // C_POSTFAIL and raised by k_post, which also advances the step counter
@@ -248,14 +240,35 @@ __global__ __launch_bounds__(256) void k_narrow_wave(const DevWorld<R>* __restri
 }
 
 
-// equations: friction, contact and constraint rows with B and 1/C
+// equations: the row offsets per world (large worlds: k_big_rowscan), then the friction and
+// contact rows of every queued pair (one pair per thread over the whole batch), then the joint
+// rows (one workgroup per world)
 template <typename R>
-__global__ __launch_bounds__(128) void k_rows(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                              double dt) {
+__global__ __launch_bounds__(128) void k_rows(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list) {
   __shared__ int sh[128];
   CHP_WORLD_PROLOGUE
-  if (W.nb >= BP_BIG) return;  // k_big_rowscan / k_big_rowbuild
-  st_rows(W, (R)dt, sh);
+  if (W.nb >= BP_BIG) return;
+  st_rowscan(W, sh);
+}
+template <typename R>
+__global__ __launch_bounds__(64) void k_pair_rows(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
+                                                  const int* __restrict__ bounds, double dt) {
+  const int n = bounds[NP_NCLS];
+  const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x), ng = (int)(gridDim.x * blockDim.x);
+  for (int i = g; i < n; i += ng) {
+    const int2 e = q[i];
+    const DevWorld<R>& W = worlds[e.x];
+    if (W.counters[C_STATUS] != 0) continue;
+    pair_rows(W, e.y, W.counters[C_FROWS], (R)dt);
+  }
+}
+template <typename R>
+__global__ __launch_bounds__(64) void k_joint_rows(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                   double dt) {
+  const DevWorld<R>& W = worlds[list[blockIdx.x]];
+  if (W.ncons == 0 || W.counters[C_STATUS] != 0) return;
+  const int F = W.counters[C_FROWS], C = W.counters[C_CROWS];
+  for (int c = threadIdx.x; c < W.ncons; c += blockDim.x) constraint_rows(W, c, F + C + W.c_row_off[c], (R)dt);
 }
 
 // Levels and GS sweeps run one large world per workgroup (the first nbig entries of the list)
@@ -990,10 +1003,14 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     k_narrow_wave<R><<<(waves + 3) / 4, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
-  k_rows<R><<<n, 128, 0, st>>>(dw, d_list, dt);
-  if (ctx->big_nw) {
-    k_big_rowscan<R><<<ctx->big_nw, 1024, 0, st>>>(dw, d_list, big_worlds(ctx));
-    k_big_rowbuild<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), dt);
+  k_rows<R><<<n, 128, 0, st>>>(dw, d_list);
+  if (ctx->big_nw) k_big_rowscan<R><<<ctx->big_nw, 1024, 0, st>>>(dw, d_list, big_worlds(ctx));
+  {
+    const int qcap = (int)ctx->d_q_cap;
+    const int* bounds = ctx->d_qcls + (size_t)n * NP_NCLS;
+    const int blocks = (std::max(64, std::min(qcap, 256 * 4 * 4 * 64)) + 63) / 64;
+    k_pair_rows<R><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_joint_rows<R><<<n, 64, 0, st>>>(dw, d_list, dt);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
   if (tune) {

@@ -832,18 +832,18 @@ template <typename R> __device__ void rows_done(const DevWorld<R>& W, int P, int
 }
 
 // rows in solver order: every friction row, every contact row, then constraint rows
-// (world_class.dart:539-635; Solver.addEquation filter, solver.dart:30-34)
-template <typename R> __device__ bool st_rows(const DevWorld<R>& W, R h, int* sh) {
-  int P = W.counters[C_PAIRS];
-  int F = array_exscan(W.pf_cnt, P, sh);
-  int C = array_exscan(W.pc_off, P, sh);
-  int E = F + C + W.ncrows;
+// (world_class.dart:539-635; Solver.addEquation filter, solver.dart:30-34).  The offsets of a
+// step: friction rows of pair p from pf_cnt[p], its contact rows from F + pc_off[p], joint rows
+// after every contact row; false (WS_ROWS) when the slab is too small
+template <typename R> __device__ bool st_rowscan(const DevWorld<R>& W, int* sh) {
+  const int P = W.counters[C_PAIRS];
+  const int F = array_exscan(W.pf_cnt, P, sh);
+  const int C = array_exscan(W.pc_off, P, sh);
+  const int E = F + C + W.ncrows;
   if (E > W.row_cap) {
     if (threadIdx.x == 0) { W.counters[C_ROWS] = E; ws_fail(W.counters, WS_ROWS); }
     return false;
   }
-  for (int p = threadIdx.x; p < P; p += blockDim.x) pair_rows(W, p, F, h);
-  for (int c = threadIdx.x; c < W.ncons; c += blockDim.x) constraint_rows(W, c, F + C + W.c_row_off[c], h);
   if (threadIdx.x == 0) rows_done(W, P, F, C, E);
   __syncthreads();
   return true;
