@@ -30,20 +30,23 @@ using namespace chp;
   const DevWorld<R>& W = worlds[list[blockIdx.x]];                  \
   if (block_bcast(W.counters[C_STATUS], sh) != 0) return;
 
-// gravity / solve mass / AABBs, broadphase and the collideConnected filter.  Dynamic LDS holds
-// the naive broadphase's body data of worlds with at most lds_nb bodies.
-template <typename R>
+// gravity / solve mass / AABBs, broadphase and the collideConnected filter.  One kernel per
+// broadphase kind (each holds only its code and registers), each skipping the other kind's
+// worlds.  Dynamic LDS holds the naive broadphase's body data of worlds with at most lds_nb
+// bodies.
+template <typename R, bool SAP>
 __global__ __launch_bounds__(256) void k_bp(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
                                             int lds_nb) {
   __shared__ int sh[256];
   extern __shared__ double lds_d[];
   CHP_WORLD_PROLOGUE
+  if ((W.broadphase == CHP_BROADPHASE_SAP) != SAP) return;
   if (threadIdx.x == 0) W.counters[C_STAGING] = 0;
   if (W.nb >= BP_BIG) return;  // the grid-wide kernels below take large worlds
-  st_prestep(W);
+  st_prestep<R, SAP>(W);
   __syncthreads();
   bool ok;
-  if (W.broadphase == CHP_BROADPHASE_SAP) {
+  if constexpr (SAP) {
     ok = st_bp_sap(W, sh);
   } else {
     BpLds<R> L;
@@ -543,6 +546,7 @@ struct chp_ctx {
   size_t d_big_cap = 0;
   std::vector<int> h_big;
   int big_nspans = 0, big_ntiles = 0, big_nw = 0, big_rounds = 0;
+  int n_sap = 0;   // SAP worlds in the current list
   int* d_status = nullptr;
   const int** d_cptrs = nullptr;
   size_t d_status_cap = 0;
@@ -563,7 +567,7 @@ struct chp_ctx {
     int nbig = 0, max_nb = 0, bl = 0, bg = 0;
     double dt = 0;
     const void *dq = nullptr, *dqcls = nullptr, *dlist = nullptr, *dworlds = nullptr, *dbig = nullptr;
-    std::array<int, 4> big{};
+    std::array<int, 5> big{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } sg;
@@ -899,8 +903,10 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, ctx->h_list.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   std::vector<int> spans, tiles, bigw;
   int max_tiles = 1;
+  ctx->n_sap = 0;
   for (size_t li = 0; li < n; li++) {
     const WorldSlot& S = ctx->worlds[list[li]];
+    if (S.broadphase == CHP_BROADPHASE_SAP) ctx->n_sap++;
     if (S.nb < BP_BIG) continue;
     bigw.push_back((int)li);
     for (int s0 = 0; s0 < S.nb; s0 += BP_SPAN) { spans.push_back((int)li); spans.push_back(s0); }
@@ -976,7 +982,8 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     // naive broadphase body data in LDS when it fits beside the static LDS (4 KiB kept free)
     const size_t per_body = BpLds<R>::bytes(1);
     const int lds_nb = std::min(max_nb, (int)((ctx->max_lds - 4096) / per_body));
-    k_bp<R><<<n, 256, BpLds<R>::bytes(lds_nb), st>>>(dw, d_list, lds_nb);
+    k_bp<R, false><<<n, 256, BpLds<R>::bytes(lds_nb), st>>>(dw, d_list, lds_nb);
+    if (ctx->n_sap) k_bp<R, true><<<n, 256, 0, st>>>(dw, d_list, 0);
     launch_big_bp<R>(ctx, d_list);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
@@ -1055,7 +1062,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
   *out = nullptr;
   if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block &&
         g.bg == ctx->solve_block && g.dq == ctx->d_q && g.dqcls == ctx->d_qcls && g.dlist == ctx->d_list &&
-        g.dbig == ctx->d_big && g.big == std::array<int, 4>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds} &&
+        g.dbig == ctx->d_big && g.big == std::array<int, 5>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap} &&
         g.dworlds == ctx->d_worlds)) {
     if (!may_capture) return CHP_OK;
     drop_graph(ctx);
@@ -1074,7 +1081,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
     g.valid = true; g.list = list; g.nbig = nbig; g.max_nb = max_nb; g.dt = dt;
     g.bl = ctx->levels_block; g.bg = ctx->solve_block;
     g.dq = ctx->d_q; g.dqcls = ctx->d_qcls; g.dlist = ctx->d_list; g.dworlds = ctx->d_worlds;
-    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds};
+    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap};
     g.graph = graph;
     g.exec = exec;
   }

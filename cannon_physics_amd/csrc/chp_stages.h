@@ -113,7 +113,8 @@ template <typename R> __device__ void body_aabb(const DevWorld<R>& W, int i) {
 
 // World.internalStep before the broadphase for body i (world_class.dart:459-471): gravity into
 // the effective force, the solve masses, cleared velocity deltas, the SAP AABB refresh
-template <typename R> __device__ __forceinline__ void prestep_body(const DevWorld<R>& W, int i) {
+// SAP: the world may be a SAP world (its dirty AABBs are refreshed); else the code is left out
+template <typename R, bool SAP = true> __device__ __forceinline__ void prestep_body(const DevWorld<R>& W, int i) {
   int ty = W.type[i];
   f3 f = F3<R>(W.force, i);
   if (ty == CHP_BODY_DYNAMIC) {
@@ -126,13 +127,15 @@ template <typename R> __device__ __forceinline__ void prestep_body(const DevWorl
   for (int k = 0; k < 9; k++) W.iiws[9 * i + k] = kin ? 0.f : W.iiw[9 * i + k];
   S3<R>(W.vl, i, mk3(0.f, 0.f, 0.f));
   S3<R>(W.wl, i, mk3(0.f, 0.f, 0.f));
-  if (W.broadphase == CHP_BROADPHASE_SAP && (W.flags[i] & BF_AABB_DIRTY)) {
-    body_aabb(W, i);
-    W.flags[i] &= ~BF_AABB_DIRTY;
+  if constexpr (SAP) {
+    if (W.broadphase == CHP_BROADPHASE_SAP && (W.flags[i] & BF_AABB_DIRTY)) {
+      body_aabb(W, i);
+      W.flags[i] &= ~BF_AABB_DIRTY;
+    }
   }
 }
-template <typename R> __device__ void st_prestep(const DevWorld<R>& W) {
-  for (int i = threadIdx.x; i < W.nb; i += blockDim.x) prestep_body(W, i);
+template <typename R, bool SAP = true> __device__ void st_prestep(const DevWorld<R>& W) {
+  for (int i = threadIdx.x; i < W.nb; i += blockDim.x) prestep_body<R, SAP>(W, i);
 }
 
 // ---- broadphase -------------------------------------------------------------------------------
@@ -178,9 +181,7 @@ __device__ __forceinline__ bool bp_test(const DevWorld<R>& W, const BpLds<R>& L,
     if ((L.gm[2 * a] & L.gm[2 * b + 1]) == 0 || (L.gm[2 * b] & L.gm[2 * a + 1]) == 0) return false;
     if (L.stat[a] && L.stat[b]) return false;
     f3 pa = mk3(L.pos[3 * a], L.pos[3 * a + 1], L.pos[3 * a + 2]), pb = mk3(L.pos[3 * b], L.pos[3 * b + 1], L.pos[3 * b + 2]);
-    f3 r = sub3<double>(pb, pa);
-    double s = (double)L.brad[a] + (double)L.brad[b];
-    return len2<double>(r) < s * s;
+    return bp_sphere_at(pa, (double)L.brad[a], pb, (double)L.brad[b]);
   } else {
     return bp_need(W, a, b) && bp_sphere(W, a, b);
   }
@@ -596,23 +597,44 @@ template <typename R> __device__ void st_narrow_place(const DevWorld<R>& W, int 
 // between the passes: offsets class-major over the listed worlds (cnt[w * NP_NCLS + c] becomes
 // the offset; bounds[c] = first entry of class c, bounds[NP_NCLS] = total).  One workgroup.
 __device__ inline void st_narrow_scan(int* cnt, int n, int* bounds, int* sh) {
-  // each thread owns a contiguous chunk of worlds; one block-wide scan per class
-  const int ch = (n + (int)blockDim.x - 1) / (int)blockDim.x;
-  const int w0 = min(n, (int)threadIdx.x * ch), w1 = min(n, w0 + ch);
-  int carry = 0;
+  // each thread owns a contiguous chunk of worlds; every class is scanned at once: within a
+  // wave by shuffles, across waves through one exchange of the wave totals in LDS
+  const int tid = threadIdx.x, bd = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = bd >> 6;
+  const int ch = (n + bd - 1) / bd;
+  const int w0 = min(n, tid * ch), w1 = min(n, w0 + ch);
+  int ex[NP_NCLS];
+#pragma unroll
   for (int c = 0; c < NP_NCLS; c++) {
-    int s = 0, tot;
+    int s = 0;
     for (int w = w0; w < w1; w++) s += cnt[w * NP_NCLS + c];
-    int ex = carry + block_exscan(s, sh, &tot);
-    for (int w = w0; w < w1; w++) {
-      int v = cnt[w * NP_NCLS + c];
-      cnt[w * NP_NCLS + c] = ex;
-      ex += v;
+    int incl = s;
+    for (int off = 1; off < 64; off <<= 1) {
+      int x = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += x;
     }
-    if (threadIdx.x == 0) bounds[c] = carry;
-    carry += tot;
+    ex[c] = incl - s;
+    if (lane == 63) sh[wave * NP_NCLS + c] = incl;
   }
-  if (threadIdx.x == 0) bounds[NP_NCLS] = carry;
+  __syncthreads();
+  int base = 0;
+#pragma unroll
+  for (int c = 0; c < NP_NCLS; c++) {
+    int woff = 0, tot = 0;
+    for (int v = 0; v < nw; v++) {
+      int t = sh[v * NP_NCLS + c];
+      woff += v < wave ? t : 0;
+      tot += t;
+    }
+    int e = base + woff + ex[c];
+    for (int w = w0; w < w1; w++) {
+      int x = cnt[w * NP_NCLS + c];
+      cnt[w * NP_NCLS + c] = e;
+      e += x;
+    }
+    if (tid == 0) bounds[c] = base;
+    base += tot;
+  }
+  if (tid == 0) bounds[NP_NCLS] = base;
 }
 
 // the resolvers of a thread class and the contacts its pairs make at most (sphere classes: one;
