@@ -334,8 +334,10 @@ __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ 
   }
 }
 
-// GS iterations
-template <typename R>
+// GS iterations.  GLOBAL = false: the worlds whose body velocities and level starts fit in
+// LDS; GLOBAL = true: the others, from global memory (a separate kernel, so that the common
+// LDS form keeps its own, smaller register count)
+template <typename R, bool GLOBAL>
 __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
                                             int nbig, int n, int big_cap) {
   __shared__ int sh[256];
@@ -348,16 +350,20 @@ __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worl
   float* lds = (float*)lds_f4;
   const int L = W.counters[C_LEVELS];
   if (wave < 0) {
-    if (W.nb <= big_cap && L + 1 <= LS_BIG)
-      st_gs<R, GBlock, true>(W, GBlock(), shd, sh, lds, (int*)(lds + 8 * big_cap));
-    else
-      st_gs<R, GBlock, false>(W, GBlock(), shd, sh, W.vw, nullptr);
+    const bool fits = W.nb <= big_cap && L + 1 <= LS_BIG;
+    if constexpr (!GLOBAL) {
+      if (fits) st_gs<R, GBlock, true>(W, GBlock(), shd, sh, lds, (int*)(lds + 8 * big_cap));
+    } else {
+      if (!fits) st_gs<R, GBlock, false>(W, GBlock(), shd, sh, W.vw, nullptr);
+    }
   } else {
+    const bool fits = W.nb <= SMALL_NB && L + 1 <= LS_SMALL;
     float* base = lds + wave * (8 * SMALL_NB + LS_SMALL);
-    if (W.nb <= SMALL_NB && L + 1 <= LS_SMALL)
-      st_gs<R, GWave, true>(W, GWave(), shd, sh, base, (int*)(base + 8 * SMALL_NB));
-    else
-      st_gs<R, GWave, false>(W, GWave(), shd, sh, W.vw, nullptr);
+    if constexpr (!GLOBAL) {
+      if (fits) st_gs<R, GWave, true>(W, GWave(), shd, sh, base, (int*)(base + 8 * SMALL_NB));
+    } else {
+      if (!fits) st_gs<R, GWave, false>(W, GWave(), shd, sh, W.vw, nullptr);
+    }
   }
 }
 
@@ -1026,7 +1032,8 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c], st));
       k_levels<R><<<groups(b), b, lds_levels(b, bcap), st>>>(dw, d_list, nbig, n, bcap);
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 1], st));
-      k_gs<R><<<groups(b), b, lds_gs(b, bcap), st>>>(dw, d_list, nbig, n, bcap);
+      k_gs<R, false><<<groups(b), b, lds_gs(b, bcap), st>>>(dw, d_list, nbig, n, bcap);
+      k_gs<R, true><<<groups(b), b, 0, st>>>(dw, d_list, nbig, n, bcap);
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 2], st));
     }
     if (ev) {
@@ -1037,7 +1044,8 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     const int bl = ctx->levels_block, bg = ctx->solve_block;
     k_levels<R><<<groups(bl), bl, lds_levels(bl, bcap), st>>>(dw, d_list, nbig, n, bcap);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
-    k_gs<R><<<groups(bg), bg, lds_gs(bg, bcap), st>>>(dw, d_list, nbig, n, bcap);
+    k_gs<R, false><<<groups(bg), bg, lds_gs(bg, bcap), st>>>(dw, d_list, nbig, n, bcap);
+    k_gs<R, true><<<groups(bg), bg, 0, st>>>(dw, d_list, nbig, n, bcap);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
   }
   if (ctx->big_nw) k_big_post<R><<<ctx->big_nspans, 64, 0, st>>>(dw, d_list, big_spans(ctx), dt);
@@ -1418,7 +1426,8 @@ static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t 
   k_solve_rows<R><<<1, 256, 0, st>>>(dw, (int)id, ctx->d_eq, (int)n, dt);
   const int bcap = big_lds_cap(nbig, S.nb, ctx->max_lds);
   k_levels<R><<<1, bl, lds_levels(bl, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap);
-  k_gs<R><<<1, bg, lds_gs(bg, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap);
+  k_gs<R, false><<<1, bg, lds_gs(bg, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap);
+  k_gs<R, true><<<1, bg, 0, st>>>(dw, ctx->d_list, nbig, 1, bcap);
   k_solve_finish<R><<<1, 256, 0, st>>>(dw, (int)id);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(st));
