@@ -235,11 +235,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                    const int* __restrict__ bounds, double dt) {
   st_narrow_class<R, CLS>(worlds, q, bounds, (R)dt);
 }
-template <typename R>
-__global__ __launch_bounds__(256) void k_narrow_wave(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
+template <typename R, int CLS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_narrow_wave(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
                                                      const int* __restrict__ bounds, double dt) {
   __shared__ f3 ws[4][NPCtx<R, true>::WS_F3];   // one LDS workspace per wave
-  st_narrow_wave(worlds, q + bounds[NP_WAVE], bounds[NP_NCLS] - bounds[NP_WAVE], (R)dt, ws[threadIdx.x >> 6]);
+  constexpr unsigned kinds = CLS == NP_WHC ? (NK_CC | NK_HC) : NK_ALL;
+  st_narrow_wave<R, kinds>(worlds, q + bounds[CLS], bounds[CLS + 1] - bounds[CLS], (R)dt, ws[threadIdx.x >> 6]);
 }
 
 
@@ -1013,7 +1014,9 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     k_narrow_cls<R, NP_CC><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
     k_narrow_cls<R, NP_OTHER><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
     const int waves = std::max(1, std::min(qcap, 4096));
-    k_narrow_wave<R><<<(waves + 3) / 4, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_wave<R, NP_WHC><<<(waves + 3) / 4, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    // compound-body pairs are rare: a small grid, strided over however many there are
+    k_narrow_wave<R, NP_WAVE><<<32, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
   k_rows<R><<<n, 128, 0, st>>>(dw, d_list);

@@ -547,7 +547,9 @@ template <typename R> __device__ void st_narrow(const DevWorld<R>& W, R dt) {
 // The batched narrowphase sorts the pairs of every listed world by resolver class, so that a
 // wave of the thread kernel runs one resolver on neighbouring data.  Classes 0..NP_WAVE-1 run one
 // pair per thread, class NP_WAVE one pair per wave.
-enum { NP_SS = 0, NP_SP, NP_SB, NP_SC, NP_SH, NP_PC, NP_CC, NP_OTHER, NP_WAVE, NP_NCLS };
+// Wave classes: NP_WHC, single-shape convex pairs needing a wave (convex–heightfield, large SAT),
+// compiled with the convex resolvers only; NP_WAVE, the rest (compound bodies).
+enum { NP_SS = 0, NP_SP, NP_SB, NP_SC, NP_SH, NP_PC, NP_CC, NP_OTHER, NP_WHC, NP_WAVE, NP_NCLS };
 template <typename R> __device__ __forceinline__ int pair_class(const DevWorld<R>& W, int bi, int bj) {
   if (W.shape_count[bi] != 1 || W.shape_count[bj] != 1) return wave_pair(W, bi, bj) ? NP_WAVE : NP_OTHER;
   int ta = W.si_type[W.shape_first[bi]], tb = W.si_type[W.shape_first[bj]];
@@ -562,8 +564,8 @@ template <typename R> __device__ __forceinline__ int pair_class(const DevWorld<R
   }
   if (lo == CHP_SHAPE_PLANE && (hi == CHP_SHAPE_BOX || hi == CHP_SHAPE_CONVEX)) return NP_PC;
   if (lo == CHP_SHAPE_BOX || lo == CHP_SHAPE_CONVEX) {
-    if (hi == CHP_SHAPE_HEIGHTFIELD) return NP_WAVE;
-    if (hi == CHP_SHAPE_BOX || hi == CHP_SHAPE_CONVEX) return wave_pair(W, bi, bj) ? NP_WAVE : NP_CC;
+    if (hi == CHP_SHAPE_HEIGHTFIELD) return NP_WHC;
+    if (hi == CHP_SHAPE_BOX || hi == CHP_SHAPE_CONVEX) return wave_pair(W, bi, bj) ? NP_WHC : NP_CC;
   }
   return NP_OTHER;
 }
@@ -657,7 +659,7 @@ __device__ void st_narrow_class(const DevWorld<R>* worlds, const int2* q, const 
 }
 
 // the queued separating-axis pairs, one wave per pair (grid-stride over the queue)
-template <typename R>
+template <typename R, unsigned KINDS>
 __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, R dt, f3* ws) {
   const int lane = threadIdx.x & 63;
   const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
@@ -668,7 +670,7 @@ __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int
     const int bi = W.pair_i[p], bj = W.pair_j[p];
     ContactRec<R> loc[CHP_KMAX];
     Sink<R, true> s{loc, CHP_KMAX, 0, 0};
-    NPCtx<R, true> ctx(W, dt, &s, ws);
+    NPCtx<R, true, KINDS> ctx(W, dt, &s, ws);
     ctx.pair(bi, bj);
     int n = s.n, base = 0;
     if (lane == 0) {
@@ -681,7 +683,7 @@ __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int
       n = 0;
     } else if (n > CHP_KMAX) {  // more contacts than the local buffer: run the pair again into the slab
       Sink<R, true> s2{W.contacts + base, n, 0, 0};
-      NPCtx<R, true> ctx2(W, dt, &s2, ws);
+      NPCtx<R, true, KINDS> ctx2(W, dt, &s2, ws);
       ctx2.pair(bi, bj);
     } else if (lane == 0) {
       for (int k = 0; k < n; k++) W.contacts[base + k] = loc[k];
