@@ -254,25 +254,28 @@ __global__ __launch_bounds__(128) void k_rows(const DevWorld<R>* __restrict__ wo
   if (W.nb >= BP_BIG) return;
   st_rowscan(W, sh);
 }
-template <typename R>
+// MODE ROWS_AB: the rows' body pairs only (before the level schedule); ROWS_GS: the rows
+// themselves, as GS records at their level-order positions (after it)
+template <typename R, int MODE>
 __global__ __launch_bounds__(64) void k_pair_rows(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
-                                                  const int* __restrict__ bounds, double dt) {
+                                                  const int* __restrict__ bounds, double dt, int keep) {
   const int n = bounds[NP_NCLS];
   const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x), ng = (int)(gridDim.x * blockDim.x);
   for (int i = g; i < n; i += ng) {
     const int2 e = q[i];
     const DevWorld<R>& W = worlds[e.x];
     if (W.counters[C_STATUS] != 0) continue;
-    pair_rows(W, e.y, W.counters[C_FROWS], (R)dt);
+    pair_rows<R, MODE>(W, e.y, W.counters[C_FROWS], (R)dt, keep != 0);
   }
 }
-template <typename R>
+template <typename R, int MODE>
 __global__ __launch_bounds__(64) void k_joint_rows(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                                   double dt) {
+                                                   double dt, int keep) {
   const DevWorld<R>& W = worlds[list[blockIdx.x]];
   if (W.ncons == 0 || W.counters[C_STATUS] != 0) return;
   const int F = W.counters[C_FROWS], C = W.counters[C_CROWS];
-  for (int c = threadIdx.x; c < W.ncons; c += blockDim.x) constraint_rows(W, c, F + C + W.c_row_off[c], (R)dt);
+  for (int c = threadIdx.x; c < W.ncons; c += blockDim.x)
+    constraint_rows<R, MODE>(W, c, F + C + W.c_row_off[c], (R)dt, keep != 0);
 }
 
 // Levels and GS sweeps run one large world per workgroup (the first nbig entries of the list)
@@ -318,7 +321,7 @@ __device__ __forceinline__ int group_world(const DevWorld<R>* worlds, const int*
 // GS dependency levels of the step's rows and the level-ordered GS records
 template <typename R>
 __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                                int nbig, int n, int big_cap) {
+                                                int nbig, int n, int big_cap, int gather) {
   __shared__ int sh[256];
   extern __shared__ int lds_i[];
   int wave;
@@ -326,12 +329,12 @@ __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ 
   if (id < 0) return;
   const DevWorld<R>& W = worlds[id];
   if (wave < 0) {
-    if (W.nb <= big_cap) st_levels<R, GBlock, true>(W, GBlock(), sh, lds_i, lds_i + big_cap, LS_BIG);
-    else st_levels<R, GBlock, false>(W, GBlock(), sh, W.tmp, nullptr, 0);
+    if (W.nb <= big_cap) st_levels<R, GBlock, true>(W, GBlock(), sh, lds_i, lds_i + big_cap, LS_BIG, gather != 0);
+    else st_levels<R, GBlock, false>(W, GBlock(), sh, W.tmp, nullptr, 0, gather != 0);
   } else {
     int* base = lds_i + wave * (SMALL_NB + LS_SMALL);
-    if (W.nb <= SMALL_NB) st_levels<R, GWave, true>(W, GWave(), sh, base, base + SMALL_NB, LS_SMALL);
-    else st_levels<R, GWave, false>(W, GWave(), sh, W.tmp, nullptr, 0);
+    if (W.nb <= SMALL_NB) st_levels<R, GWave, true>(W, GWave(), sh, base, base + SMALL_NB, LS_SMALL, gather != 0);
+    else st_levels<R, GWave, false>(W, GWave(), sh, W.tmp, nullptr, 0, gather != 0);
   }
 }
 
@@ -340,7 +343,7 @@ __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ 
 // LDS form keeps its own, smaller register count)
 template <typename R, bool GLOBAL>
 __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                            int nbig, int n, int big_cap) {
+                                            int nbig, int n, int big_cap, int keep) {
   __shared__ int sh[256];
   __shared__ double shd[256];
   extern __shared__ float4 lds_f4[];
@@ -353,17 +356,17 @@ __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worl
   if (wave < 0) {
     const bool fits = W.nb <= big_cap && L + 1 <= LS_BIG;
     if constexpr (!GLOBAL) {
-      if (fits) st_gs<R, GBlock, true>(W, GBlock(), shd, sh, lds, (int*)(lds + 8 * big_cap));
+      if (fits) st_gs<R, GBlock, true>(W, GBlock(), shd, sh, lds, (int*)(lds + 8 * big_cap), keep != 0);
     } else {
-      if (!fits) st_gs<R, GBlock, false>(W, GBlock(), shd, sh, W.vw, nullptr);
+      if (!fits) st_gs<R, GBlock, false>(W, GBlock(), shd, sh, W.vw, nullptr, keep != 0);
     }
   } else {
     const bool fits = W.nb <= SMALL_NB && L + 1 <= LS_SMALL;
     float* base = lds + wave * (8 * SMALL_NB + LS_SMALL);
     if constexpr (!GLOBAL) {
-      if (fits) st_gs<R, GWave, true>(W, GWave(), shd, sh, base, (int*)(base + 8 * SMALL_NB));
+      if (fits) st_gs<R, GWave, true>(W, GWave(), shd, sh, base, (int*)(base + 8 * SMALL_NB), keep != 0);
     } else {
-      if (!fits) st_gs<R, GWave, false>(W, GWave(), shd, sh, W.vw, nullptr);
+      if (!fits) st_gs<R, GWave, false>(W, GWave(), shd, sh, W.vw, nullptr, keep != 0);
     }
   }
 }
@@ -554,6 +557,7 @@ struct chp_ctx {
   std::vector<int> h_big;
   int big_nspans = 0, big_ntiles = 0, big_nw = 0, big_rounds = 0;
   int n_sap = 0;   // SAP worlds in the current list
+  bool keep_rows = false;   // steps also write row records (the capture reads them)
   int* d_status = nullptr;
   const int** d_cptrs = nullptr;
   size_t d_status_cap = 0;
@@ -572,6 +576,7 @@ struct chp_ctx {
     bool valid = false;
     std::vector<int> list;
     int nbig = 0, max_nb = 0, bl = 0, bg = 0;
+    bool keep = false;
     double dt = 0;
     const void *dq = nullptr, *dqcls = nullptr, *dlist = nullptr, *dworlds = nullptr, *dbig = nullptr;
     std::array<int, 5> big{};
@@ -968,8 +973,8 @@ template <typename R> static void launch_big_bp(chp_ctx* ctx, const int* d_list)
   k_big_rows<R><<<4 * ns, 256, 0, st>>>(dw, d_list, spans, 1);
 }
 
-enum { K_BP = 0, K_NARROW, K_ROWS, K_LEVELS, K_GS, K_POST, K_NSTAGE };
-static const char* kStageNames[K_NSTAGE] = {"k_bp", "k_narrow", "k_rows", "k_levels", "k_gs", "k_post"};
+enum { K_BP = 0, K_NARROW, K_ROWS, K_LEVELS, K_GSROWS, K_GS, K_POST, K_NSTAGE };
+static const char* kStageNames[K_NSTAGE] = {"k_bp", "k_narrow", "k_rows", "k_levels", "k_gsrows", "k_gs", "k_post"};
 
 // launches the stage kernels of one step for the worlds in d_list[0..n); ev (optional) gets
 // K_NSTAGE + 1 events around them
@@ -1019,41 +1024,49 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     k_narrow_wave<R, NP_WAVE><<<32, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
+  // rows: offsets, then the rows' body pairs; the level schedule; then the rows, written as GS
+  // records in level order (keep: also as row records, for the capture)
+  const int keep = ctx->keep_rows ? 1 : 0;
+  const int* qbounds = ctx->d_qcls + (size_t)n * NP_NCLS;
+  const int qblocks = (std::max(64, std::min((int)ctx->d_q_cap, 256 * 4 * 4 * 64)) + 63) / 64;
+  auto build_rows = [&]() {
+    k_pair_rows<R, ROWS_GS><<<qblocks, 64, 0, st>>>(dw, ctx->d_q, qbounds, dt, keep);
+    k_joint_rows<R, ROWS_GS><<<n, 64, 0, st>>>(dw, d_list, dt, keep);
+  };
   k_rows<R><<<n, 128, 0, st>>>(dw, d_list);
   if (ctx->big_nw) k_big_rowscan<R><<<ctx->big_nw, 1024, 0, st>>>(dw, d_list, big_worlds(ctx));
-  {
-    const int qcap = (int)ctx->d_q_cap;
-    const int* bounds = ctx->d_qcls + (size_t)n * NP_NCLS;
-    const int blocks = (std::max(64, std::min(qcap, 256 * 4 * 4 * 64)) + 63) / 64;
-    k_pair_rows<R><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
-    k_joint_rows<R><<<n, 64, 0, st>>>(dw, d_list, dt);
-  }
+  k_pair_rows<R, ROWS_AB><<<qblocks, 64, 0, st>>>(dw, ctx->d_q, qbounds, dt, 0);
+  k_joint_rows<R, ROWS_AB><<<n, 64, 0, st>>>(dw, d_list, dt, 0);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
   if (tune) {
     for (int c = 0; c < 3; c++) {
       const int b = kTuneBlocks[c];
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c], st));
-      k_levels<R><<<groups(b), b, lds_levels(b, bcap), st>>>(dw, d_list, nbig, n, bcap);
+      k_levels<R><<<groups(b), b, lds_levels(b, bcap), st>>>(dw, d_list, nbig, n, bcap, 0);
+      build_rows();   // at the positions this schedule gave (the order inside a level may differ)
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 1], st));
-      k_gs<R, false><<<groups(b), b, lds_gs(b, bcap), st>>>(dw, d_list, nbig, n, bcap);
-      k_gs<R, true><<<groups(b), b, 0, st>>>(dw, d_list, nbig, n, bcap);
+      k_gs<R, false><<<groups(b), b, lds_gs(b, bcap), st>>>(dw, d_list, nbig, n, bcap, keep);
+      k_gs<R, true><<<groups(b), b, 0, st>>>(dw, d_list, nbig, n, bcap, keep);
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 2], st));
     }
     if (ev) {
       HIPCHK(ctx, hipEventRecord(ev[4], st));
       HIPCHK(ctx, hipEventRecord(ev[5], st));
+      HIPCHK(ctx, hipEventRecord(ev[6], st));
     }
   } else {
     const int bl = ctx->levels_block, bg = ctx->solve_block;
-    k_levels<R><<<groups(bl), bl, lds_levels(bl, bcap), st>>>(dw, d_list, nbig, n, bcap);
+    k_levels<R><<<groups(bl), bl, lds_levels(bl, bcap), st>>>(dw, d_list, nbig, n, bcap, 0);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
-    k_gs<R, false><<<groups(bg), bg, lds_gs(bg, bcap), st>>>(dw, d_list, nbig, n, bcap);
-    k_gs<R, true><<<groups(bg), bg, 0, st>>>(dw, d_list, nbig, n, bcap);
+    build_rows();
     if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
+    k_gs<R, false><<<groups(bg), bg, lds_gs(bg, bcap), st>>>(dw, d_list, nbig, n, bcap, keep);
+    k_gs<R, true><<<groups(bg), bg, 0, st>>>(dw, d_list, nbig, n, bcap, keep);
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
   }
   if (ctx->big_nw) k_big_post<R><<<ctx->big_nspans, 64, 0, st>>>(dw, d_list, big_spans(ctx), dt);
   k_post<R><<<n, 256, 0, st>>>(dw, d_list, dt);
-  if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
+  if (ev) HIPCHK(ctx, hipEventRecord(ev[7], st));
   HIPCHK(ctx, hipGetLastError());
   return CHP_OK;
 }
@@ -1071,7 +1084,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
                       hipGraphExec_t* out) {
   chp_ctx::StepGraph& g = ctx->sg;
   *out = nullptr;
-  if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block &&
+  if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block && g.keep == ctx->keep_rows &&
         g.bg == ctx->solve_block && g.dq == ctx->d_q && g.dqcls == ctx->d_qcls && g.dlist == ctx->d_list &&
         g.dbig == ctx->d_big && g.big == std::array<int, 5>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap} &&
         g.dworlds == ctx->d_worlds)) {
@@ -1090,7 +1103,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
       return fail(ctx, CHP_E_HIP, std::string("step graph: ") + hipGetErrorString(e));
     }
     g.valid = true; g.list = list; g.nbig = nbig; g.max_nb = max_nb; g.dt = dt;
-    g.bl = ctx->levels_block; g.bg = ctx->solve_block;
+    g.bl = ctx->levels_block; g.bg = ctx->solve_block; g.keep = ctx->keep_rows;
     g.dq = ctx->d_q; g.dqcls = ctx->d_qcls; g.dlist = ctx->d_list; g.dworlds = ctx->d_worlds;
     g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap};
     g.graph = graph;
@@ -1313,7 +1326,9 @@ template <typename R> static int broadphase_t(chp_ctx* ctx, uint32_t id, int32_t
 template <typename R>
 static int capture_t(chp_ctx* ctx, uint32_t id, double dt, int32_t* pi, int32_t* pj, uint32_t pcap, uint32_t* np,
                      chp_row* rows, uint32_t rcap, uint32_t* nr, int32_t* iters) {
+  ctx->keep_rows = true;
   int r = step_plain_t<R>(ctx, id, dt, 1);
+  ctx->keep_rows = false;
   if (r != CHP_OK) return r;
   DevWorld<R>& D = dev_of<R>(ctx->worlds[id]);
   int c[C_COUNT];
@@ -1428,9 +1443,9 @@ static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t 
   hipStream_t st = ctx->stream;
   k_solve_rows<R><<<1, 256, 0, st>>>(dw, (int)id, ctx->d_eq, (int)n, dt);
   const int bcap = big_lds_cap(nbig, S.nb, ctx->max_lds);
-  k_levels<R><<<1, bl, lds_levels(bl, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap);
-  k_gs<R, false><<<1, bg, lds_gs(bg, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap);
-  k_gs<R, true><<<1, bg, 0, st>>>(dw, ctx->d_list, nbig, 1, bcap);
+  k_levels<R><<<1, bl, lds_levels(bl, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap, 1);
+  k_gs<R, false><<<1, bg, lds_gs(bg, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap, 1);
+  k_gs<R, true><<<1, bg, 0, st>>>(dw, ctx->d_list, nbig, 1, bcap, 1);
   k_solve_finish<R><<<1, 256, 0, st>>>(dw, (int)id);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(st));

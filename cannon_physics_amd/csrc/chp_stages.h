@@ -713,9 +713,25 @@ template <typename R> __device__ R giMf(const DevWorld<R>& W, int a, int b, f3 s
 template <typename R> __device__ R gwv(const DevWorld<R>& W, int a, int b, f3 sa, f3 ra, f3 sb, f3 rb) {
   return mulvec<R>(F3<R>(W.vel, a), F3<R>(W.avel, a), sa, ra) + mulvec<R>(F3<R>(W.vel, b), F3<R>(W.avel, b), sb, rb);
 }
+// Where the row builders put a row r: ROWS_REC, its row record (and body pair); ROWS_AB, only
+// its body pair (the level schedule needs no more); ROWS_GS, its GS record, straight at its
+// level-order position rpos[r] (plus the row record when keep).
+enum : int { ROWS_REC = 0, ROWS_AB = 1, ROWS_GS = 2 };
+template <typename R> struct GsWord;
+template <typename R>
+__device__ __forceinline__ void gs_store(typename GsWord<R>::T* base, int k, const RowRec<R>& r, unsigned fa, unsigned fb);
+
 // stores the row with 1/C (Equation.computeC = computeGiMGt + eps, :130-174)
-template <typename R> __device__ void put_row(const DevWorld<R>& W, int r, int kind, int a, int b, f3 sa, f3 ra, f3 sb,
-                                              f3 rb, R B, R eps, R minF, R maxF) {
+template <typename R, int MODE = ROWS_REC>
+__device__ void put_row(const DevWorld<R>& W, int r, int kind, int a, int b, f3 sa, f3 ra, f3 sb, f3 rb, R B, R eps,
+                        R minF, R maxF, bool keep = false) {
+  const int fa = a | (W.type[a] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
+  const int fb = b | (W.type[b] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
+  if constexpr (MODE == ROWS_AB) {
+    W.row_ab[2 * r] = fa;
+    W.row_ab[2 * r + 1] = fb;
+    return;
+  }
   RowRec<R> o;
   o.a = a; o.b = b; o.kind = kind; o.pad = 0;
   o.sa = sa; o.ra = ra; o.sb = sb; o.rb = rb;
@@ -730,13 +746,21 @@ template <typename R> __device__ void put_row(const DevWorld<R>& W, int r, int k
   o.lambda = 0;
   o.minF = minF; o.maxF = maxF;
   o.ima = W.ims[a]; o.imb = W.ims[b];
-  W.rows[r] = o;
-  W.row_ab[2 * r] = a | (W.type[a] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
-  W.row_ab[2 * r + 1] = b | (W.type[b] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
+  if constexpr (MODE == ROWS_GS) {
+    const int k = W.rpos[r];
+    gs_store<R>((typename GsWord<R>::T*)W.gsr, k, o, (unsigned)fa, (unsigned)fb);
+    W.glam[k] = (R)0;
+    if (keep) W.rows[r] = o;
+  } else {
+    W.rows[r] = o;
+    W.row_ab[2 * r] = fa;
+    W.row_ab[2 * r + 1] = fb;
+  }
 }
 // ContactEquation.computeB (contact_equation.dart:34-77)
-template <typename R> __device__ void contact_row(const DevWorld<R>& W, int r, int a, int b, f3 ri, f3 rj, f3 n, R rest,
-                                                  R sa_, R sb_, R se, R h, R minF, R maxF) {
+template <typename R, int MODE = ROWS_REC>
+__device__ void contact_row(const DevWorld<R>& W, int r, int a, int b, f3 ri, f3 rj, f3 n, R rest,
+                                                  R sa_, R sb_, R se, R h, R minF, R maxF, bool keep = false) {
   f3 rixn = cross3<R>(ri, n), rjxn = cross3<R>(rj, n);
   f3 gsa = neg3(n), gra = neg3(rixn), gsb = n, grb = rjxn;
   f3 pen = add3<R>(F3<R>(W.pos, b), rj);
@@ -748,42 +772,46 @@ template <typename R> __device__ void contact_row(const DevWorld<R>& W, int r, i
          dot3<R>(F3<R>(W.avel, a), rixn);
   R gm = giMf(W, a, b, gsa, gra, gsb, grb);
   R B = -g * sa_ - gw * sb_ - h * gm;
-  put_row(W, r, CHP_ROW_CONTACT, a, b, gsa, gra, gsb, grb, B, se, minF, maxF);
+  put_row<R, MODE>(W, r, CHP_ROW_CONTACT, a, b, gsa, gra, gsb, grb, B, se, minF, maxF, keep);
 }
 // FrictionEquation.computeB (friction_equation.dart:19-48)
-template <typename R> __device__ void friction_row(const DevWorld<R>& W, int r, int a, int b, f3 ri, f3 rj, f3 t, R sb_,
-                                                   R se, R h, R minF, R maxF) {
+template <typename R, int MODE = ROWS_REC>
+__device__ void friction_row(const DevWorld<R>& W, int r, int a, int b, f3 ri, f3 rj, f3 t, R sb_,
+                                                   R se, R h, R minF, R maxF, bool keep = false) {
   f3 rixt = cross3<R>(ri, t), rjxt = cross3<R>(rj, t);
   f3 gsa = neg3(t), gra = neg3(rixt), gsb = t, grb = rjxt;
   R gw = gwv(W, a, b, gsa, gra, gsb, grb);
   R gm = giMf(W, a, b, gsa, gra, gsb, grb);
   R B = -gw * sb_ - h * gm;
-  put_row(W, r, CHP_ROW_FRICTION, a, b, gsa, gra, gsb, grb, B, se, minF, maxF);
+  put_row<R, MODE>(W, r, CHP_ROW_FRICTION, a, b, gsa, gra, gsb, grb, B, se, minF, maxF, keep);
 }
 // RotationalEquation / ConeEquation.computeB (rotational_equation.dart:34-58, cone_equation.dart:34-57)
-template <typename R> __device__ void rot_row(const DevWorld<R>& W, int r, int kind, int a, int b, f3 ni, f3 nj, R cosA,
-                                              R h, R minF, R maxF, R sa_, R sb_, R se) {
+template <typename R, int MODE = ROWS_REC>
+__device__ void rot_row(const DevWorld<R>& W, int r, int kind, int a, int b, f3 ni, f3 nj, R cosA, R h, R minF, R maxF,
+                        R sa_, R sb_, R se, bool keep = false) {
   f3 z = mk3(0.f, 0.f, 0.f);
   f3 gra = cross3<R>(nj, ni), grb = cross3<R>(ni, nj);
   R g = cosA - dot3<R>(ni, nj);
   R gw = gwv(W, a, b, z, gra, z, grb);
   R gm = giMf(W, a, b, z, gra, z, grb);
   R B = -g * sa_ - gw * sb_ - h * gm;
-  put_row(W, r, kind, a, b, z, gra, z, grb, B, se, minF, maxF);
+  put_row<R, MODE>(W, r, kind, a, b, z, gra, z, grb, B, se, minF, maxF, keep);
 }
 // RotationalMotorEquation.computeB (rotational_motor_equation.dart:17-33)
-template <typename R> __device__ void motor_row(const DevWorld<R>& W, int r, int a, int b, f3 axA, f3 axB, R target, R h,
-                                                R minF, R maxF, R sb_, R se) {
+template <typename R, int MODE = ROWS_REC>
+__device__ void motor_row(const DevWorld<R>& W, int r, int a, int b, f3 axA, f3 axB, R target, R h, R minF, R maxF,
+                          R sb_, R se, bool keep = false) {
   f3 z = mk3(0.f, 0.f, 0.f);
   f3 gra = axA, grb = neg3(axB);
   R gw = gwv(W, a, b, z, gra, z, grb) - target;
   R gm = giMf(W, a, b, z, gra, z, grb);
   R B = -gw * sb_ - h * gm;
-  put_row(W, r, CHP_ROW_MOTOR, a, b, z, gra, z, grb, B, se, minF, maxF);
+  put_row<R, MODE>(W, r, CHP_ROW_MOTOR, a, b, z, gra, z, grb, B, se, minF, maxF, keep);
 }
 
 // Constraint.update + the rows each joint feeds the solver (lib/constraints/)
-template <typename R> __device__ void constraint_rows(const DevWorld<R>& W, int c, int row0, R h) {
+template <typename R, int MODE = ROWS_REC>
+__device__ void constraint_rows(const DevWorld<R>& W, int c, int row0, R h, bool keep = false) {
   int type = W.c_type[c], a = W.c_a[c], b = W.c_b[c], mask = W.c_eq_mask[c];
   if (!mask) return;
   const f3* v = (const f3*)W.c_vec + 12 * c;
@@ -795,37 +823,38 @@ template <typename R> __device__ void constraint_rows(const DevWorld<R>& W, int 
     R half = eq[24] * (R)0.5;
     f3 n = normalize3<R>(sub3<R>(F3<R>(W.pos, b), F3<R>(W.pos, a)));
     f3 ri = scale3<R>(n, half), rj = scale3<R>(n, -half);
-    contact_row(W, r, a, b, ri, rj, n, (R)0, sa_, sb_, se, h, eq[0], eq[1]);
+    contact_row<R, MODE>(W, r, a, b, ri, rj, n, (R)0, sa_, sb_, se, h, eq[0], eq[1], keep);
     return;
   }
   // PointToPointConstraint.update (point_to_point_constraint.dart:68-83): three ContactEquations
   f3 ri = qrot<R>(qa, v[0]), rj = qrot<R>(qb, v[1]);
   const f3 axes3[3] = {mk3(1.f, 0.f, 0.f), mk3(0.f, 1.f, 0.f), mk3(0.f, 0.f, 1.f)};
-  for (int k = 0; k < 3; k++) contact_row(W, r++, a, b, ri, rj, axes3[k], (R)0, sa_, sb_, se, h, eq[4 * k], eq[4 * k + 1]);
+  for (int k = 0; k < 3; k++) contact_row<R, MODE>(W, r++, a, b, ri, rj, axes3[k], (R)0, sa_, sb_, se, h, eq[4 * k], eq[4 * k + 1], keep);
   if (type == CHP_CONSTRAINT_HINGE) {  // hinge_constraint.dart:79-104
     f3 wA = qrot<R>(qa, v[2]), wB = qrot<R>(qb, v[3]);
     f3 t1, t2;
     tangents3<R>(wA, t1, t2);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, t1, wB, eq[12 + 2], h, eq[12], eq[13], sa_, sb_, se);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, t2, wB, eq[16 + 2], h, eq[16], eq[17], sa_, sb_, se);
-    if (mask & 32) motor_row(W, r++, a, b, qrot<R>(qa, v[2]), qrot<R>(qb, v[3]), eq[20 + 3], h, eq[20], eq[21], sb_, se);
+    rot_row<R, MODE>(W, r++, CHP_ROW_ROTATIONAL, a, b, t1, wB, eq[12 + 2], h, eq[12], eq[13], sa_, sb_, se, keep);
+    rot_row<R, MODE>(W, r++, CHP_ROW_ROTATIONAL, a, b, t2, wB, eq[16 + 2], h, eq[16], eq[17], sa_, sb_, se, keep);
+    if (mask & 32) motor_row<R, MODE>(W, r++, a, b, qrot<R>(qa, v[2]), qrot<R>(qb, v[3]), eq[20 + 3], h, eq[20], eq[21], sb_, se, keep);
   } else if (type == CHP_CONSTRAINT_LOCK) {  // lock_constraint.dart:70-88
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[4]), qrot<R>(qb, v[7]), eq[14], h, eq[12], eq[13], sa_, sb_, se);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[6]), qrot<R>(qb, v[9]), eq[18], h, eq[16], eq[17], sa_, sb_, se);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[8]), qrot<R>(qb, v[5]), eq[22], h, eq[20], eq[21], sa_, sb_, se);
+    rot_row<R, MODE>(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[4]), qrot<R>(qb, v[7]), eq[14], h, eq[12], eq[13], sa_, sb_, se, keep);
+    rot_row<R, MODE>(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[6]), qrot<R>(qb, v[9]), eq[18], h, eq[16], eq[17], sa_, sb_, se, keep);
+    rot_row<R, MODE>(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, v[8]), qrot<R>(qb, v[5]), eq[22], h, eq[20], eq[21], sa_, sb_, se, keep);
   } else if (type == CHP_CONSTRAINT_CONE_TWIST) {  // cone_twist_constraint.dart:77-96
     f3 cA = qrot<R>(qa, v[2]), cB = qrot<R>(qb, v[3]);
-    rot_row(W, r++, CHP_ROW_CONE, a, b, cA, cB, eq[14], h, eq[12], eq[13], sa_, sb_, se);
+    rot_row<R, MODE>(W, r++, CHP_ROW_CONE, a, b, cA, cB, eq[14], h, eq[12], eq[13], sa_, sb_, se, keep);
     f3 t1, t2, u1, u2;
     tangents3<R>(v[2], t1, t2);   // axisA.tangents(twist.axisA, twist.axisA): the second write wins
     tangents3<R>(v[3], u1, u2);
-    rot_row(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, t2), qrot<R>(qb, u2), eq[18], h, eq[16], eq[17], sa_, sb_, se);
+    rot_row<R, MODE>(W, r++, CHP_ROW_ROTATIONAL, a, b, qrot<R>(qa, t2), qrot<R>(qb, u2), eq[18], h, eq[16], eq[17], sa_, sb_, se, keep);
   }
 }
 
 // the friction rows (from the scanned offset pf_cnt[p]) and contact rows (from F + pc_off[p]) of
 // pair p's contacts
-template <typename R> __device__ void pair_rows(const DevWorld<R>& W, int p, int F, R h) {
+template <typename R, int MODE = ROWS_REC>
+__device__ void pair_rows(const DevWorld<R>& W, int p, int F, R h, bool keep = false) {
   int fo = W.pf_cnt[p], co = F + W.pc_off[p];
   for (int k = 0; k < W.pc_cnt[p]; k++) {
     const ContactRec<R>& c = W.contacts[W.pc_base[p] + k];
@@ -836,12 +865,12 @@ template <typename R> __device__ void pair_rows(const DevWorld<R>& W, int p, int
       spook<R>(cm[4], cm[5], h, fa, fb, fe);
       f3 t1, t2;
       tangents3<R>(c.ni, t1, t2);
-      friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t1, fb, fe, h, -c.slip, c.slip);
-      friction_row(W, fo++, c.bi, c.bj, c.ri, c.rj, t2, fb, fe, h, -c.slip, c.slip);
+      friction_row<R, MODE>(W, fo++, c.bi, c.bj, c.ri, c.rj, t1, fb, fe, h, -c.slip, c.slip, keep);
+      friction_row<R, MODE>(W, fo++, c.bi, c.bj, c.ri, c.rj, t2, fb, fe, h, -c.slip, c.slip, keep);
     }
     R ca, cb, ce;
     spook<R>(cm[2], cm[3], h, ca, cb, ce);
-    contact_row(W, co++, c.bi, c.bj, c.ri, c.rj, c.ni, c.restitution, ca, cb, ce, h, (R)0, (R)1e6);
+    contact_row<R, MODE>(W, co++, c.bi, c.bj, c.ri, c.rj, c.ni, c.restitution, ca, cb, ce, h, (R)0, (R)1e6, keep);
   }
 }
 // the row counters of a step (thread 0), after the offsets are known
@@ -1140,8 +1169,11 @@ template <typename R> __device__ __forceinline__ void gs_store(typename GsWord<R
 //         or not LDS)
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// gather: the rows are in their row records and go to GS records here; else the row builders
+// write the GS records afterwards, at the positions rpos this leaves
 template <typename R, typename G, bool LDS>
-__device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int* sh, int* last, int* hist, int hist_cap) {
+__device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int* sh, int* last, int* hist, int hist_cap,
+                                          bool gather) {
   const int E = uni(W.counters[C_ROWS]), nb = W.nb;
   const int tid = g.tid, bd = g.n;
   const int2* ab = (const int2*)W.row_ab;
@@ -1233,14 +1265,16 @@ __device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int*
   };
   if (LDS && L + 1 <= hist_cap) sort(hist);
   else sort(W.last_level);
-  typedef typename GsWord<R>::T T;
-  T* gsr = (T*)W.gsr;
-  for (int k = tid; k < E; k += bd) {
-    int r = W.sched[k];
-    RowRec<R> o = W.rows[r];
-    int2 x = ab[r];
-    gs_store<R>(gsr, k, o, (unsigned)x.x, (unsigned)x.y);
-    W.glam[k] = (R)0;
+  if (gather) {
+    typedef typename GsWord<R>::T T;
+    T* gsr = (T*)W.gsr;
+    for (int k = tid; k < E; k += bd) {
+      int r = W.sched[k];
+      RowRec<R> o = W.rows[r];
+      int2 x = ab[r];
+      gs_store<R>(gsr, k, o, (unsigned)x.x, (unsigned)x.y);
+      W.glam[k] = (R)0;
+    }
   }
   if (tid == 0) W.counters[C_LEVELS] = L;
   g.sync();
@@ -1319,7 +1353,8 @@ __device__ __forceinline__ R gs_row(const GsRow<R>& cr, R clam, bool v, int ks, 
 // a row whose lambda changes before the slot is used, and the new lambda is forwarded to it.
 // lam / adl have one dummy slot at row_cap for idle lanes.
 template <typename R, typename G, bool LDS>
-__device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* shd, int* sh, float* vw, int* ls_lds) {
+__device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* shd, int* sh, float* vw, int* ls_lds,
+                                      bool keep) {
   constexpr int D = sizeof(R) == 4 ? 4 : 2;
   const int E = uni(W.counters[C_ROWS]), L = uni(W.counters[C_LEVELS]), iters = uni(W.iterations);
   const int nb = W.nb, tid = g.tid, bd = g.n, dummy = W.row_cap;
@@ -1428,8 +1463,9 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
     }
   swept:
     g.sync();
-    // Equation.multiplier and the capture read lambda in solver order
-    for (int k = tid; k < E; k += bd) W.rows[W.sched[k]].lambda = lam[k];
+    // Equation.multiplier and the capture read lambda from the row records (keep)
+    if (keep)
+      for (int k = tid; k < E; k += bd) W.rows[W.sched[k]].lambda = lam[k];
   }
   g.sync();
   // hand the lambda velocities to st_post
