@@ -139,19 +139,19 @@ __device__ __forceinline__ void big_pair_classes(const DevWorld<R>& W, int2 sp, 
 }
 template <typename R>
 __global__ __launch_bounds__(256) void k_big_ncount(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                                    const int2* __restrict__ spans, int* __restrict__ qcls) {
+                                                    const int2* __restrict__ spans, int* __restrict__ qcls, int n) {
   __shared__ int sh[NP_NCLS];
   CHP_SPAN_PROLOGUE(spans)
   if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = 0;
   __syncthreads();
   big_pair_classes(W, sp, sh, true);
   __syncthreads();
-  if (threadIdx.x < NP_NCLS && sh[threadIdx.x]) atomicAdd(&qcls[sp.x * NP_NCLS + threadIdx.x], sh[threadIdx.x]);
+  if (threadIdx.x < NP_NCLS && sh[threadIdx.x]) atomicAdd(&qcls[threadIdx.x * n + sp.x], sh[threadIdx.x]);
 }
 template <typename R>
 __global__ __launch_bounds__(256) void k_big_nplace(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
                                                     const int2* __restrict__ spans, int* __restrict__ qcls,
-                                                    int2* __restrict__ q) {
+                                                    int2* __restrict__ q, int n) {
   __shared__ int sh[NP_NCLS];
   CHP_SPAN_PROLOGUE(spans)
   if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = 0;
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_big_nplace(const DevWorld<R>* __restric
   big_pair_classes(W, sp, sh, false);
   __syncthreads();
   if (threadIdx.x < NP_NCLS)
-    sh[threadIdx.x] = sh[threadIdx.x] ? atomicAdd(&qcls[sp.x * NP_NCLS + threadIdx.x], sh[threadIdx.x]) : 0;
+    sh[threadIdx.x] = sh[threadIdx.x] ? atomicAdd(&qcls[threadIdx.x * n + sp.x], sh[threadIdx.x]) : 0;
   __syncthreads();
   const int K = (W.nb + BP_SPAN - 1) / BP_SPAN, k = sp.y / BP_SPAN, P = W.counters[C_PAIRS], wid = list[sp.x];
   for (int p = k * 256 + (int)threadIdx.x; p < P; p += K * 256) {
@@ -204,19 +204,19 @@ __global__ __launch_bounds__(64) void k_big_post(const DevWorld<R>* __restrict__
 
 // narrowphase: per-class pair counts per world, class-major offsets, placement in the queue,
 // then one pair per thread over the thread classes and one pair per wave over the wave class,
-// each grid-wide over the whole batch.  qcls: NP_NCLS ints per listed world; bounds:
+// each grid-wide over the whole batch.  qcls: class-major, qcls[c * n + list index]; bounds:
 // NP_NCLS + 1 ints.
 template <typename R>
 __global__ __launch_bounds__(64) void k_narrow_count(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                                     int* __restrict__ qcls) {
+                                                     int* __restrict__ qcls, int n) {
   __shared__ int sh[64];
-  int* cnt = qcls + (size_t)blockIdx.x * NP_NCLS;
+  int* cnt = qcls + blockIdx.x;
   const DevWorld<R>& W = worlds[list[blockIdx.x]];
   if (block_bcast(W.counters[C_STATUS], sh) != 0 || W.nb >= BP_BIG) {  // large worlds: k_big_ncount
-    if (threadIdx.x < NP_NCLS) cnt[threadIdx.x] = 0;
+    if (threadIdx.x < NP_NCLS) cnt[threadIdx.x * n] = 0;
     return;
   }
-  st_narrow_count(W, cnt, sh);
+  st_narrow_count(W, cnt, n, sh);
 }
 __global__ __launch_bounds__(1024) void k_narrow_scan(int* __restrict__ qcls, int n, int* __restrict__ bounds) {
   __shared__ int sh[1024];
@@ -224,11 +224,11 @@ __global__ __launch_bounds__(1024) void k_narrow_scan(int* __restrict__ qcls, in
 }
 template <typename R>
 __global__ __launch_bounds__(64) void k_narrow(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                               const int* __restrict__ qcls, int2* __restrict__ q) {
+                                               const int* __restrict__ qcls, int2* __restrict__ q, int n) {
   __shared__ int sh[64];
   CHP_WORLD_PROLOGUE
   if (W.nb >= BP_BIG) return;  // k_big_nplace
-  st_narrow_place(W, list[blockIdx.x], qcls + (size_t)blockIdx.x * NP_NCLS, q, sh);
+  st_narrow_place(W, list[blockIdx.x], qcls + blockIdx.x, n, q, sh);
 }
 template <typename R, int CLS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_narrow_cls(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
@@ -1002,12 +1002,12 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
   {
     const int qcap = (int)ctx->d_q_cap;
     int* bounds = ctx->d_qcls + (size_t)n * NP_NCLS;
-    k_narrow_count<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls);
-    if (ctx->big_nw) k_big_ncount<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls);
+    k_narrow_count<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls, n);
+    if (ctx->big_nw) k_big_ncount<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls, n);
     k_narrow_scan<<<1, 1024, 0, st>>>(ctx->d_qcls, n, bounds);
-    k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls, ctx->d_q);
+    k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls, ctx->d_q, n);
     if (ctx->big_nw)
-      k_big_nplace<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls, ctx->d_q);
+      k_big_nplace<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls, ctx->d_q, n);
     // one kernel per thread class (each compiles only its resolver), up to 4 waves per SIMD
     const int blocks = (std::max(64, std::min(qcap, 256 * 4 * 4 * 64)) + 63) / 64;
     k_narrow_cls<R, NP_SS><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);

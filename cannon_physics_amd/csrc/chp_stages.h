@@ -1,6 +1,7 @@
-// chp_stages.h -- the stages of World.internalStep (world_class.dart:433-701) as workgroup-wide
-// device functions.  One workgroup owns one world; the batched kernel runs every stage of
-// every step back to back for its world, the plain path launches one kernel per stage.
+// chp_stages.h -- the stages of World.internalStep (world_class.dart:433-701) as device
+// functions.  Per-world stages run on the workgroup (or wave) that owns a world; per-pair and
+// per-body stages of the whole batch, and of large worlds, run over the grid.  The kernels in
+// chp_api.hip launch each stage once over every listed world.
 #pragma once
 #include "chp_narrow.h"
 
@@ -570,8 +571,8 @@ template <typename R> __device__ __forceinline__ int pair_class(const DevWorld<R
   return NP_OTHER;
 }
 
-// pass 1: per-class pair counts of the group's world (cnt: NP_NCLS ints of the group)
-template <typename R> __device__ void st_narrow_count(const DevWorld<R>& W, int* cnt, int* sh) {
+// pass 1: per-class pair counts of the group's world (class c's count at cnt[c * stride])
+template <typename R> __device__ void st_narrow_count(const DevWorld<R>& W, int* cnt, int stride, int* sh) {
   int P = W.counters[C_PAIRS];
   if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = 0;
   __syncthreads();
@@ -581,13 +582,14 @@ template <typename R> __device__ void st_narrow_count(const DevWorld<R>& W, int*
     atomicAdd(&sh[pair_class(W, bi, bj)], 1);
   }
   __syncthreads();
-  if (threadIdx.x < NP_NCLS) cnt[threadIdx.x] = sh[threadIdx.x];
+  if (threadIdx.x < NP_NCLS) cnt[threadIdx.x * stride] = sh[threadIdx.x];
 }
 
-// pass 2: the world's pairs into the queue at its offsets (off: NP_NCLS ints of the group)
-template <typename R> __device__ void st_narrow_place(const DevWorld<R>& W, int wid, const int* off, int2* q, int* sh) {
+// pass 2: the world's pairs into the queue at its offsets (class c's at off[c * stride])
+template <typename R>
+__device__ void st_narrow_place(const DevWorld<R>& W, int wid, const int* off, int stride, int2* q, int* sh) {
   int P = W.counters[C_PAIRS];
-  if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = off[threadIdx.x];
+  if (threadIdx.x < NP_NCLS) sh[threadIdx.x] = off[threadIdx.x * stride];
   __syncthreads();
   for (int p = threadIdx.x; p < P; p += blockDim.x) {
     int bi = W.pair_i[p], bj = W.pair_j[p];
@@ -596,47 +598,66 @@ template <typename R> __device__ void st_narrow_place(const DevWorld<R>& W, int 
   }
 }
 
-// between the passes: offsets class-major over the listed worlds (cnt[w * NP_NCLS + c] becomes
-// the offset; bounds[c] = first entry of class c, bounds[NP_NCLS] = total).  One workgroup.
+// between the passes: offsets class-major over the listed worlds.  cnt[c * n + w] holds world
+// w's count of class c and becomes its offset; bounds[c] = first entry of class c,
+// bounds[NP_NCLS] = total.  One workgroup: the class totals first, then rounds of bd worlds
+// (lane t of round j takes world j * bd + t: coalesced), each scanned at once for every class
+// by wave shuffles and one exchange of the wave totals.
 __device__ inline void st_narrow_scan(int* cnt, int n, int* bounds, int* sh) {
-  // each thread owns a contiguous chunk of worlds; every class is scanned at once: within a
-  // wave by shuffles, across waves through one exchange of the wave totals in LDS
   const int tid = threadIdx.x, bd = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = bd >> 6;
-  const int ch = (n + bd - 1) / bd;
-  const int w0 = min(n, tid * ch), w1 = min(n, w0 + ch);
-  int ex[NP_NCLS];
+  int base[NP_NCLS];
+  {
+    int part[NP_NCLS];
 #pragma unroll
-  for (int c = 0; c < NP_NCLS; c++) {
-    int s = 0;
-    for (int w = w0; w < w1; w++) s += cnt[w * NP_NCLS + c];
-    int incl = s;
-    for (int off = 1; off < 64; off <<= 1) {
-      int x = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += x;
-    }
-    ex[c] = incl - s;
-    if (lane == 63) sh[wave * NP_NCLS + c] = incl;
-  }
-  __syncthreads();
-  int base = 0;
+    for (int c = 0; c < NP_NCLS; c++) part[c] = 0;
+    for (int w = tid; w < n; w += bd)
 #pragma unroll
-  for (int c = 0; c < NP_NCLS; c++) {
-    int woff = 0, tot = 0;
-    for (int v = 0; v < nw; v++) {
-      int t = sh[v * NP_NCLS + c];
-      woff += v < wave ? t : 0;
-      tot += t;
+      for (int c = 0; c < NP_NCLS; c++) part[c] += cnt[c * n + w];
+#pragma unroll
+    for (int c = 0; c < NP_NCLS; c++) {
+      for (int off = 32; off > 0; off >>= 1) part[c] += __shfl_xor(part[c], off, 64);
+      if (lane == 0) sh[wave * NP_NCLS + c] = part[c];
     }
-    int e = base + woff + ex[c];
-    for (int w = w0; w < w1; w++) {
-      int x = cnt[w * NP_NCLS + c];
-      cnt[w * NP_NCLS + c] = e;
-      e += x;
+    __syncthreads();
+    int run = 0;
+#pragma unroll
+    for (int c = 0; c < NP_NCLS; c++) {
+      int tot = 0;
+      for (int v = 0; v < nw; v++) tot += sh[v * NP_NCLS + c];
+      base[c] = run;
+      if (tid == 0) bounds[c] = run;
+      run += tot;
     }
-    if (tid == 0) bounds[c] = base;
-    base += tot;
+    if (tid == 0) bounds[NP_NCLS] = run;
+    __syncthreads();
   }
-  if (tid == 0) bounds[NP_NCLS] = base;
+  for (int w0 = 0; w0 < n; w0 += bd) {
+    const int w = w0 + tid;
+    int v[NP_NCLS], incl[NP_NCLS];
+#pragma unroll
+    for (int c = 0; c < NP_NCLS; c++) {
+      v[c] = w < n ? cnt[c * n + w] : 0;
+      incl[c] = v[c];
+      for (int off = 1; off < 64; off <<= 1) {
+        int x = __shfl_up(incl[c], off, 64);
+        if (lane >= off) incl[c] += x;
+      }
+      if (lane == 63) sh[wave * NP_NCLS + c] = incl[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NP_NCLS; c++) {
+      int woff = 0, tot = 0;
+      for (int u = 0; u < nw; u++) {
+        int t = sh[u * NP_NCLS + c];
+        woff += u < wave ? t : 0;
+        tot += t;
+      }
+      if (w < n) cnt[c * n + w] = base[c] + woff + incl[c] - v[c];
+      base[c] += tot;
+    }
+    __syncthreads();
+  }
 }
 
 // the resolvers of a thread class and the contacts its pairs make at most (sphere classes: one;
