@@ -722,17 +722,39 @@ template <typename R> __device__ __forceinline__ void spook(R k, R d, R h, R& a,
 }
 // JacobianElement.multiplyVectors
 template <typename R> __device__ __forceinline__ R mulvec(f3 s, f3 r, f3 gs, f3 gr) { return dot3<R>(s, gs) + dot3<R>(r, gr); }
+// The body state the row builders read (Equation.bi / bj), loaded once per contact or joint and
+// shared by its rows: the row stores do not alias it, but the compiler cannot know that.
+template <typename R> struct RowBody {
+  int i, fl;            // index; with GS_DYN when dynamic
+  f3 p, v, w, f, t;     // position, velocity, angular velocity, effective force, torque
+  R im;                 // invMassSolve
+  float I[9];           // invInertiaWorldSolve
+};
+template <typename R> __device__ __forceinline__ RowBody<R> row_body(const DevWorld<R>& W, int i) {
+  RowBody<R> o;
+  o.i = i;
+  o.fl = i | (W.type[i] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
+  o.p = F3<R>(W.pos, i);
+  o.v = F3<R>(W.vel, i);
+  o.w = F3<R>(W.avel, i);
+  o.f = F3<R>(W.feff, i);
+  o.t = F3<R>(W.torque, i);
+  o.im = W.ims[i];
+#pragma unroll
+  for (int k = 0; k < 9; k++) o.I[k] = W.iiws[9 * i + k];
+  return o;
+}
 // Equation.computeGiMf (equation_class.dart:108-127)
-template <typename R> __device__ R giMf(const DevWorld<R>& W, int a, int b, f3 sa, f3 ra, f3 sb, f3 rb) {
-  f3 fa = scale3<R>(F3<R>(W.feff, a), W.ims[a]);
-  f3 fb = scale3<R>(F3<R>(W.feff, b), W.ims[b]);
-  f3 ta = mvmul<R>(W.iiws + 9 * a, F3<R>(W.torque, a));
-  f3 tb = mvmul<R>(W.iiws + 9 * b, F3<R>(W.torque, b));
+template <typename R> __device__ __forceinline__ R giMf(const RowBody<R>& A, const RowBody<R>& B, f3 sa, f3 ra, f3 sb, f3 rb) {
+  f3 fa = scale3<R>(A.f, A.im);
+  f3 fb = scale3<R>(B.f, B.im);
+  f3 ta = mvmul<R>(A.I, A.t);
+  f3 tb = mvmul<R>(B.I, B.t);
   return mulvec<R>(fa, ta, sa, ra) + mulvec<R>(fb, tb, sb, rb);
 }
 // Equation.computeGW (:82-92)
-template <typename R> __device__ R gwv(const DevWorld<R>& W, int a, int b, f3 sa, f3 ra, f3 sb, f3 rb) {
-  return mulvec<R>(F3<R>(W.vel, a), F3<R>(W.avel, a), sa, ra) + mulvec<R>(F3<R>(W.vel, b), F3<R>(W.avel, b), sb, rb);
+template <typename R> __device__ __forceinline__ R gwv(const RowBody<R>& A, const RowBody<R>& B, f3 sa, f3 ra, f3 sb, f3 rb) {
+  return mulvec<R>(A.v, A.w, sa, ra) + mulvec<R>(B.v, B.w, sb, rb);
 }
 // Where the row builders put a row r: ROWS_REC, its row record (and body pair); ROWS_AB, only
 // its body pair (the level schedule needs no more); ROWS_GS, its GS record, straight at its
@@ -744,29 +766,28 @@ __device__ __forceinline__ void gs_store(typename GsWord<R>::T* base, int k, con
 
 // stores the row with 1/C (Equation.computeC = computeGiMGt + eps, :130-174)
 template <typename R, int MODE = ROWS_REC>
-__device__ void put_row(const DevWorld<R>& W, int r, int kind, int a, int b, f3 sa, f3 ra, f3 sb, f3 rb, R B, R eps,
-                        R minF, R maxF, bool keep = false) {
-  const int fa = a | (W.type[a] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
-  const int fb = b | (W.type[b] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
+__device__ void put_row(const DevWorld<R>& W, int r, int kind, const RowBody<R>& A, const RowBody<R>& Bd, f3 sa, f3 ra,
+                        f3 sb, f3 rb, R B, R eps, R minF, R maxF, bool keep = false) {
+  const int fa = A.fl, fb = Bd.fl;
   if constexpr (MODE == ROWS_AB) {
     W.row_ab[2 * r] = fa;
     W.row_ab[2 * r + 1] = fb;
     return;
   }
   RowRec<R> o;
-  o.a = a; o.b = b; o.kind = kind; o.pad = 0;
+  o.a = A.i; o.b = Bd.i; o.kind = kind; o.pad = 0;
   o.sa = sa; o.ra = ra; o.sb = sb; o.rb = rb;
-  R gim = W.ims[a] + W.ims[b];
-  o.ia = mvmul<R>(W.iiws + 9 * a, ra);
+  R gim = A.im + Bd.im;
+  o.ia = mvmul<R>(A.I, ra);
   gim += dot3<R>(o.ia, ra);
-  o.ib = mvmul<R>(W.iiws + 9 * b, rb);
+  o.ib = mvmul<R>(Bd.I, rb);
   gim += dot3<R>(o.ib, rb);
   o.B = B;
   o.invC = (R)1 / (gim + eps);
   o.eps = eps;
   o.lambda = 0;
   o.minF = minF; o.maxF = maxF;
-  o.ima = W.ims[a]; o.imb = W.ims[b];
+  o.ima = A.im; o.imb = Bd.im;
   if constexpr (MODE == ROWS_GS) {
     const int k = W.rpos[r];
     gs_store<R>((typename GsWord<R>::T*)W.gsr, k, o, (unsigned)fa, (unsigned)fb);
@@ -780,52 +801,51 @@ __device__ void put_row(const DevWorld<R>& W, int r, int kind, int a, int b, f3 
 }
 // ContactEquation.computeB (contact_equation.dart:34-77)
 template <typename R, int MODE = ROWS_REC>
-__device__ void contact_row(const DevWorld<R>& W, int r, int a, int b, f3 ri, f3 rj, f3 n, R rest,
-                                                  R sa_, R sb_, R se, R h, R minF, R maxF, bool keep = false) {
+__device__ void contact_row(const DevWorld<R>& W, int r, const RowBody<R>& a, const RowBody<R>& b, f3 ri, f3 rj, f3 n,
+                            R rest, R sa_, R sb_, R se, R h, R minF, R maxF, bool keep = false) {
   f3 rixn = cross3<R>(ri, n), rjxn = cross3<R>(rj, n);
   f3 gsa = neg3(n), gra = neg3(rixn), gsb = n, grb = rjxn;
-  f3 pen = add3<R>(F3<R>(W.pos, b), rj);
-  pen = sub3<R>(pen, F3<R>(W.pos, a));
+  f3 pen = add3<R>(b.p, rj);
+  pen = sub3<R>(pen, a.p);
   pen = sub3<R>(pen, ri);
   R g = dot3<R>(n, pen);
   R e1 = rest + (R)1;
-  R gw = e1 * dot3<R>(F3<R>(W.vel, b), n) - e1 * dot3<R>(F3<R>(W.vel, a), n) + dot3<R>(F3<R>(W.avel, b), rjxn) -
-         dot3<R>(F3<R>(W.avel, a), rixn);
-  R gm = giMf(W, a, b, gsa, gra, gsb, grb);
+  R gw = e1 * dot3<R>(b.v, n) - e1 * dot3<R>(a.v, n) + dot3<R>(b.w, rjxn) - dot3<R>(a.w, rixn);
+  R gm = giMf(a, b, gsa, gra, gsb, grb);
   R B = -g * sa_ - gw * sb_ - h * gm;
   put_row<R, MODE>(W, r, CHP_ROW_CONTACT, a, b, gsa, gra, gsb, grb, B, se, minF, maxF, keep);
 }
 // FrictionEquation.computeB (friction_equation.dart:19-48)
 template <typename R, int MODE = ROWS_REC>
-__device__ void friction_row(const DevWorld<R>& W, int r, int a, int b, f3 ri, f3 rj, f3 t, R sb_,
-                                                   R se, R h, R minF, R maxF, bool keep = false) {
+__device__ void friction_row(const DevWorld<R>& W, int r, const RowBody<R>& a, const RowBody<R>& b, f3 ri, f3 rj, f3 t,
+                             R sb_, R se, R h, R minF, R maxF, bool keep = false) {
   f3 rixt = cross3<R>(ri, t), rjxt = cross3<R>(rj, t);
   f3 gsa = neg3(t), gra = neg3(rixt), gsb = t, grb = rjxt;
-  R gw = gwv(W, a, b, gsa, gra, gsb, grb);
-  R gm = giMf(W, a, b, gsa, gra, gsb, grb);
+  R gw = gwv(a, b, gsa, gra, gsb, grb);
+  R gm = giMf(a, b, gsa, gra, gsb, grb);
   R B = -gw * sb_ - h * gm;
   put_row<R, MODE>(W, r, CHP_ROW_FRICTION, a, b, gsa, gra, gsb, grb, B, se, minF, maxF, keep);
 }
 // RotationalEquation / ConeEquation.computeB (rotational_equation.dart:34-58, cone_equation.dart:34-57)
 template <typename R, int MODE = ROWS_REC>
-__device__ void rot_row(const DevWorld<R>& W, int r, int kind, int a, int b, f3 ni, f3 nj, R cosA, R h, R minF, R maxF,
-                        R sa_, R sb_, R se, bool keep = false) {
+__device__ void rot_row(const DevWorld<R>& W, int r, int kind, const RowBody<R>& a, const RowBody<R>& b, f3 ni, f3 nj,
+                        R cosA, R h, R minF, R maxF, R sa_, R sb_, R se, bool keep = false) {
   f3 z = mk3(0.f, 0.f, 0.f);
   f3 gra = cross3<R>(nj, ni), grb = cross3<R>(ni, nj);
   R g = cosA - dot3<R>(ni, nj);
-  R gw = gwv(W, a, b, z, gra, z, grb);
-  R gm = giMf(W, a, b, z, gra, z, grb);
+  R gw = gwv(a, b, z, gra, z, grb);
+  R gm = giMf(a, b, z, gra, z, grb);
   R B = -g * sa_ - gw * sb_ - h * gm;
   put_row<R, MODE>(W, r, kind, a, b, z, gra, z, grb, B, se, minF, maxF, keep);
 }
 // RotationalMotorEquation.computeB (rotational_motor_equation.dart:17-33)
 template <typename R, int MODE = ROWS_REC>
-__device__ void motor_row(const DevWorld<R>& W, int r, int a, int b, f3 axA, f3 axB, R target, R h, R minF, R maxF,
-                          R sb_, R se, bool keep = false) {
+__device__ void motor_row(const DevWorld<R>& W, int r, const RowBody<R>& a, const RowBody<R>& b, f3 axA, f3 axB,
+                          R target, R h, R minF, R maxF, R sb_, R se, bool keep = false) {
   f3 z = mk3(0.f, 0.f, 0.f);
   f3 gra = axA, grb = neg3(axB);
-  R gw = gwv(W, a, b, z, gra, z, grb) - target;
-  R gm = giMf(W, a, b, z, gra, z, grb);
+  R gw = gwv(a, b, z, gra, z, grb) - target;
+  R gm = giMf(a, b, z, gra, z, grb);
   R B = -gw * sb_ - h * gm;
   put_row<R, MODE>(W, r, CHP_ROW_MOTOR, a, b, z, gra, z, grb, B, se, minF, maxF, keep);
 }
@@ -833,16 +853,17 @@ __device__ void motor_row(const DevWorld<R>& W, int r, int a, int b, f3 axA, f3 
 // Constraint.update + the rows each joint feeds the solver (lib/constraints/)
 template <typename R, int MODE = ROWS_REC>
 __device__ void constraint_rows(const DevWorld<R>& W, int c, int row0, R h, bool keep = false) {
-  int type = W.c_type[c], a = W.c_a[c], b = W.c_b[c], mask = W.c_eq_mask[c];
+  int type = W.c_type[c], ia = W.c_a[c], ib = W.c_b[c], mask = W.c_eq_mask[c];
   if (!mask) return;
+  const RowBody<R> a = row_body(W, ia), b = row_body(W, ib);
   const f3* v = (const f3*)W.c_vec + 12 * c;
   const R* eq = W.c_eq + 25 * c;
-  f4 qa = ((const f4*)W.quat)[a], qb = ((const f4*)W.quat)[b];
+  f4 qa = ((const f4*)W.quat)[ia], qb = ((const f4*)W.quat)[ib];
   int r = row0;
   const R sa_ = W.c_spook[0], sb_ = W.c_spook[1], se = W.c_spook[2];
   if (type == CHP_CONSTRAINT_DISTANCE) {  // distance_constraint.dart:27-38
     R half = eq[24] * (R)0.5;
-    f3 n = normalize3<R>(sub3<R>(F3<R>(W.pos, b), F3<R>(W.pos, a)));
+    f3 n = normalize3<R>(sub3<R>(b.p, a.p));
     f3 ri = scale3<R>(n, half), rj = scale3<R>(n, -half);
     contact_row<R, MODE>(W, r, a, b, ri, rj, n, (R)0, sa_, sb_, se, h, eq[0], eq[1], keep);
     return;
@@ -877,21 +898,27 @@ __device__ void constraint_rows(const DevWorld<R>& W, int c, int row0, R h, bool
 template <typename R, int MODE = ROWS_REC>
 __device__ void pair_rows(const DevWorld<R>& W, int p, int F, R h, bool keep = false) {
   int fo = W.pf_cnt[p], co = F + W.pc_off[p];
-  for (int k = 0; k < W.pc_cnt[p]; k++) {
+  const int n = W.pc_cnt[p];
+  if (n == 0) return;
+  // a pair's contacts mostly share their body order (a compound pair's may not)
+  RowBody<R> A = row_body(W, W.contacts[W.pc_base[p]].bi), B = row_body(W, W.contacts[W.pc_base[p]].bj);
+  for (int k = 0; k < n; k++) {
     const ContactRec<R>& c = W.contacts[W.pc_base[p] + k];
     if (!(c.flags & 1)) continue;
+    if (c.bi != A.i) A = row_body(W, c.bi);
+    if (c.bj != B.i) B = row_body(W, c.bj);
     const R* cm = W.cm_params + 6 * c.cm;
     if (c.flags & 2) {
       R fa, fb, fe;
       spook<R>(cm[4], cm[5], h, fa, fb, fe);
       f3 t1, t2;
       tangents3<R>(c.ni, t1, t2);
-      friction_row<R, MODE>(W, fo++, c.bi, c.bj, c.ri, c.rj, t1, fb, fe, h, -c.slip, c.slip, keep);
-      friction_row<R, MODE>(W, fo++, c.bi, c.bj, c.ri, c.rj, t2, fb, fe, h, -c.slip, c.slip, keep);
+      friction_row<R, MODE>(W, fo++, A, B, c.ri, c.rj, t1, fb, fe, h, -c.slip, c.slip, keep);
+      friction_row<R, MODE>(W, fo++, A, B, c.ri, c.rj, t2, fb, fe, h, -c.slip, c.slip, keep);
     }
     R ca, cb, ce;
     spook<R>(cm[2], cm[3], h, ca, cb, ce);
-    contact_row<R, MODE>(W, co++, c.bi, c.bj, c.ri, c.rj, c.ni, c.restitution, ca, cb, ce, h, (R)0, (R)1e6, keep);
+    contact_row<R, MODE>(W, co++, A, B, c.ri, c.rj, c.ni, c.restitution, ca, cb, ce, h, (R)0, (R)1e6, keep);
   }
 }
 // the row counters of a step (thread 0), after the offsets are known
@@ -995,14 +1022,17 @@ template <typename R> __device__ void st_solve_rows(const DevWorld<R>& W, const 
       const f3 ri = mk3(e.ri[0], e.ri[1], e.ri[2]), rj = mk3(e.rj[0], e.rj[1], e.rj[2]);
       const f3 nn = mk3(e.n[0], e.n[1], e.n[2]);
       const R sa_ = (R)e.a, sb_ = (R)e.b, se = (R)e.eps, mn = (R)e.min_force, mx = (R)e.max_force;
-      if (pass == 0 && e.kind == CHP_ROW_CONTACT)
-        contact_row(W, r, e.body_i, e.body_j, ri, rj, nn, (R)e.restitution, sa_, sb_, se, h, mn, mx);
-      else if (pass == 1 && e.kind == CHP_ROW_FRICTION)
-        friction_row(W, r, e.body_i, e.body_j, ri, rj, nn, sb_, se, h, mn, mx);
+      const int ps = e.kind == CHP_ROW_CONTACT ? 0 : e.kind == CHP_ROW_FRICTION ? 1 : e.kind == CHP_ROW_MOTOR ? 3 : 2;
+      if (ps != pass) continue;
+      const RowBody<R> A = row_body(W, e.body_i), B = row_body(W, e.body_j);
+      if (pass == 0)
+        contact_row(W, r, A, B, ri, rj, nn, (R)e.restitution, sa_, sb_, se, h, mn, mx);
+      else if (pass == 1)
+        friction_row(W, r, A, B, ri, rj, nn, sb_, se, h, mn, mx);
       else if (pass == 2 && (e.kind == CHP_ROW_ROTATIONAL || e.kind == CHP_ROW_CONE))
-        rot_row(W, r, e.kind, e.body_i, e.body_j, ri, rj, (R)e.angle_cos, h, mn, mx, sa_, sb_, se);
-      else if (pass == 3 && e.kind == CHP_ROW_MOTOR)
-        motor_row(W, r, e.body_i, e.body_j, ri, rj, (R)e.target_velocity, h, mn, mx, sb_, se);
+        rot_row(W, r, e.kind, A, B, ri, rj, (R)e.angle_cos, h, mn, mx, sa_, sb_, se);
+      else if (pass == 3)
+        motor_row(W, r, A, B, ri, rj, (R)e.target_velocity, h, mn, mx, sb_, se);
     }
     __syncthreads();
   }
