@@ -992,8 +992,8 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
   if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
   {
     // naive broadphase body data in LDS when it fits beside the static LDS (4 KiB kept free)
-    const size_t per_body = BpLds<R>::bytes(1);
-    const int lds_nb = std::min(max_nb, (int)((ctx->max_lds - 4096) / per_body));
+    int lds_nb = max_nb;
+    while (lds_nb > 0 && BpLds<R>::bytes(lds_nb) > ctx->max_lds - 4096) lds_nb--;
     k_bp<R, false><<<n, 256, BpLds<R>::bytes(lds_nb), st>>>(dw, d_list, lds_nb);
     if (ctx->n_sap) k_bp<R, true><<<n, 256, 0, st>>>(dw, d_list, 0);
     launch_big_bp<R>(ctx, d_list);

@@ -162,15 +162,26 @@ __device__ __forceinline__ float axis_of(f3 v, int ax) { return ax == 0 ? v.x : 
 // consecutive j at a time, a ballot gives each passing lane its rank, so pairs come out in j
 // order.  Pass 1 counts per row, a scan gives the row offsets, pass 2 writes.  The body data
 // the tests read (position, bounding radius, filter group / mask, static flag) is staged in LDS
-// when the world fits (BpLds), else read from the slab.
+// when the world fits (BpLds), else read from the slab; in LDS, pass 1 also keeps every chunk's
+// ballot mask, and pass 2 writes from the masks instead of testing again.
+// Row i has ceil(i / 64) chunks; the masks of rows 0..i-1 take bp_chunks(i) words.
+__host__ __device__ __forceinline__ int bp_chunks(int i) {
+  if (i <= 1) return 0;
+  const int a = (i - 1) >> 6, b = (i - 1) & 63;
+  return (a + 1) * (32 * a + b);
+}
 template <typename R> struct BpLds {
+  unsigned long long* masks;   // bp_chunks(n) words
   float* pos;  // 3 per body
   R* brad;
   int* gm;     // group, mask per body
   int* stat;   // 1: static
-  static __host__ __device__ size_t bytes(int n) { return (size_t)n * (3 * sizeof(float) + sizeof(R) + 3 * sizeof(int)); }
+  static __host__ __device__ size_t bytes(int n) {
+    return (size_t)bp_chunks(n) * sizeof(unsigned long long) + (size_t)n * (3 * sizeof(float) + sizeof(R) + 3 * sizeof(int));
+  }
   __device__ void carve(void* base, int cap) {
-    brad = (R*)base;
+    masks = (unsigned long long*)base;
+    brad = (R*)(masks + bp_chunks(cap));
     pos = (float*)(brad + cap);
     gm = (int*)(pos + 3 * cap);
     stat = gm + 2 * cap;
@@ -208,7 +219,13 @@ template <typename R, bool LDS> __device__ bool st_bp_naive(const DevWorld<R>& W
   }
   for (int i = wave; i < n; i += nwave) {
     int c = 0;
-    for (int j0 = 0; j0 < i; j0 += 64) c += __popcll(bp_row_chunk<R, LDS>(W, L, i, j0));
+    for (int j0 = 0; j0 < i; j0 += 64) {
+      const unsigned long long m = bp_row_chunk<R, LDS>(W, L, i, j0);
+      if constexpr (LDS) {
+        if (lane == 0) L.masks[bp_chunks(i) + (j0 >> 6)] = m;
+      }
+      c += __popcll(m);
+    }
     if (lane == 0) W.cnt[i] = c;
   }
   __syncthreads();
@@ -220,7 +237,9 @@ template <typename R, bool LDS> __device__ bool st_bp_naive(const DevWorld<R>& W
   for (int i = wave; i < n; i += nwave) {
     int o = W.cnt[i];
     for (int j0 = 0; j0 < i; j0 += 64) {
-      unsigned long long m = bp_row_chunk<R, LDS>(W, L, i, j0);
+      unsigned long long m;
+      if constexpr (LDS) m = L.masks[bp_chunks(i) + (j0 >> 6)];
+      else m = bp_row_chunk<R, LDS>(W, L, i, j0);
       if ((m >> lane) & 1ull) {
         int k = o + __popcll(m & ((1ull << lane) - 1ull));
         W.pair_i[k] = i;
