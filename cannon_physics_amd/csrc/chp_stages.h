@@ -34,19 +34,40 @@ __device__ __forceinline__ int block_exscan(int v, int* sh, int* total) {
   return incl - v;
 }
 // in-place exclusive scan of data[0..n); returns the total (same value in every thread).  Each
-// thread scans a contiguous chunk: two passes over the data, one block-wide scan of the chunk
-// sums.
+// wave scans a contiguous segment in coalesced rounds of 64 (a shuffle scan per round, a
+// running carry); a pass of wave sums ahead gives every segment its offset.  sh needs
+// blockDim.x / 64 + 1 ints; blockDim.x is a multiple of 64.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int x = __shfl_up(v, off, 64);
+    if (lane >= off) v += x;
+  }
+  return v;
+}
 __device__ __forceinline__ int array_exscan(int* data, int n, int* sh) {
-  const int c = (n + (int)blockDim.x - 1) / (int)blockDim.x;
-  const int b = min(n, (int)threadIdx.x * c), e = min(n, b + c);
+  const int nw = (int)(blockDim.x >> 6), wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int seg = (((n + nw - 1) / nw) + 63) & ~63;
+  const int b = min(n, wv * seg), e = min(n, b + seg);
   int s = 0;
-  for (int i = b; i < e; i++) s += data[i];
-  int tot;
-  int ex = block_exscan(s, sh, &tot);
-  for (int i = b; i < e; i++) {
-    int v = data[i];
-    data[i] = ex;
-    ex += v;
+  for (int i = b + lane; i < e; i += 64) s += data[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) sh[wv] = s;
+  __syncthreads();
+  int carry = 0, tot = 0;
+  for (int w = 0; w < nw; w++) {
+    const int x = sh[w];
+    carry += w < wv ? x : 0;
+    tot += x;
+  }
+  for (int i0 = b; i0 < e; i0 += 64) {
+    const int i = i0 + lane;
+    const int v = i < e ? data[i] : 0;
+    const int inc = wave_incl_scan(v);
+    if (i < e) data[i] = carry + inc - v;
+    carry += __shfl(inc, 63, 64);
   }
   __syncthreads();
   return tot;
