@@ -289,12 +289,14 @@ def main():
         % (K, wall, dev_ms, P, Cn, E))
     log("stage ms over the timed region: " + json.dumps({k: round(v, 2) for k, v in stages.items()}))
 
-    # the same number of steps again without per-stage events: chp_step replays a captured
-    # graph of the step (captured, with the tuned launch shape, by one untimed call first)
+    # the same number of steps again without per-stage events, from a captured graph of the
+    # step (captured, with the tuned launch shape, by one untimed call first; graphs are opt-in)
+    ctx.set_graphs(True)
     ctx.step(wids, dt, 2)
     t0 = time.perf_counter()
     ctx.step(wids, dt, K)
     graph_wall = time.perf_counter() - t0
+    ctx.set_graphs(False)
     log("graph replay: %d steps in %.3fs" % (K, graph_wall))
     dom = max(stages, key=stages.get)
     gs_ms = stages["k_gs"] / K

@@ -583,7 +583,7 @@ struct chp_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } sg;
-  bool use_graphs = true;
+  bool use_graphs = false;       // captured step graphs (chp_ctx_set_graphs)
   std::vector<hipEvent_t> tev;               // 3 events per candidate
   int2* d_q = nullptr;           // narrowphase queue of a step (world, pair), sorted by resolver class
   size_t d_q_cap = 0;

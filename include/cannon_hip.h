@@ -263,8 +263,9 @@ int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int levels_block);
  * (every run leaves the same result, so the step is the step chp_step would take). */
 int chp_ctx_autotune(chp_ctx* ctx);
 int chp_ctx_get_tuning(chp_ctx* ctx, int* solve_block, int* levels_block);
-/* chp_step with nsteps > 1 captures one step's kernels in a hipGraph and replays it for each
- * step (on by default; off: every step is launched kernel by kernel).  Same results either way. */
+/* on: chp_step with nsteps > 1 captures one step's kernels in a hipGraph and replays it for each
+ * step; off (the default): every step is launched kernel by kernel, which measured as fast or
+ * faster on every workload tried.  Same results either way. */
 int chp_ctx_set_graphs(chp_ctx* ctx, int on);
 /* per-stage HIP events in chp_step (costs one event per stage kernel) */
 int chp_ctx_set_profiling(chp_ctx* ctx, int on);

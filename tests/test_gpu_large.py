@@ -74,7 +74,9 @@ def test_large_and_small_worlds_in_one_batch_equal_each_alone():
     batch = [c.create_world(r) for r in recs]
     alone = [c.create_world(r) for r in recs]
     try:
+        c.set_graphs(True)
         c.step(batch, DT, 8)            # captured step graph, replayed
+        c.set_graphs(False)
         for w in alone:
             c.step_plain(w, DT, 8)      # one world per launch, kernel by kernel
         for a, b in zip(batch, alone):

@@ -240,6 +240,7 @@ def test_graph_replays_equal_eager_steps():
     between, equal the kernel-by-kernel path bit for bit after every call."""
     recs = [scenes.c3_world(i).recipe() for i in (310, 311, 312)] + [scenes.c4_chain_world(19).recipe()]
     g, e = Context(device=0), Context(device=0)
+    g.set_graphs(True)
     e.set_graphs(False)
     try:
         gi = [g.create_world(r) for r in recs]
