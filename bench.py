@@ -191,32 +191,95 @@ def serve_bench(ntemplates, nrequests, budget_gib, tick_steps, seed=5, max_submi
                       "steps, dt 1/60, FP32" % (rate, ntemplates)}
 
 
-def cpu_baseline(recs, dyn, dt, warm, stride, target_s=15.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(recs, dyn, dt, warm, stride, target_s=6.0, runs=3):
     """The oracle (reference semantics, FP64 arithmetic) on every stride-th world, one
-    std::thread per world slice, on the host cores; the number of timed steps is chosen so that
-    the timed region takes about target_s seconds."""
+    std::thread per world slice on every host core: one warm-up run, then the median of `runs`
+    timed runs, each of about target_s seconds (the steps per run are sized from the warm-up)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     L = oracle_lib.lib("f64")
     sample = list(range(0, len(recs), stride))
     ws = [L.orc_create(recs[i].ptr) for i in sample]
     arr = (C.c_void_p * len(ws))(*ws)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, os.cpu_count() or 1)
     t0 = time.perf_counter()
     L.orc_step_many(arr, len(ws), dt, warm, threads)
     per_step = (time.perf_counter() - t0) / max(1, warm)
     steps = int(max(3, min(600, target_s / max(per_step, 1e-6))))
-    t0 = time.perf_counter()
-    L.orc_step_many(arr, len(ws), dt, steps, threads)
-    el = time.perf_counter() - t0
+    rates = []
     nb = sum(dyn[i] for i in sample)
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        L.orc_step_many(arr, len(ws), dt, steps, threads)
+        rates.append(nb * steps / (time.perf_counter() - t0))
     for w in ws:
         L.orc_destroy(w)
-    return {"value": nb * steps / el, "unit": "body-steps/s", "cores": threads, "kind": "port",
-            "sample": "%d of the %d worlds (every %d-th), %d dynamic bodies, %d steps after %d warm-up "
-                      "steps, %.1f s wall; C++ restatement of the reference semantics (FP64 "
-                      "arithmetic over FP32 storage), one std::thread per world slice"
-                      % (len(sample), len(recs), stride, nb, steps, warm, el)}
+    rates.sort()
+    return {"value": rates[len(rates) // 2], "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "runs": rates,
+            "sample": "%d of the %d worlds (every %d-th), %d dynamic bodies, %d warm-up steps, then the "
+                      "median of %d runs of %d steps; C++ restatement of the reference semantics (FP64 "
+                      "arithmetic over FP32 storage), one std::thread per world slice on %d host threads"
+                      % (len(sample), len(recs), stride, nb, warm, runs, steps, threads)}
+
+
+def precision_report(ctx, recs, wids, dyn, dt, preroll, steps_total, stride):
+    """FP32 device path against the reference's own arithmetic (FP64 expressions over FP32
+    storage: the device's F64 mode, which the parity tests hold bit-identical to the oracle's
+    FP64 mode) on every stride-th world of the batch:
+      one step from identical state -- max |dx|, max |dv|, and the worlds whose contact count
+      differs (a narrowphase decision that flipped);
+      drift -- both paths from the initial state over the config's step count, max |dx|."""
+    import numpy as np
+    from cannon_physics_amd import _abi as A
+    from cannon_physics_amd.world import Context, state_to_numpy
+    sample = list(range(0, len(recs), stride))
+    c64 = Context(device=0, precision=A.PRECISION_F64)
+    w64 = [c64.create_world(recs[i]) for i in sample]
+    w32 = [wids[i] for i in sample]
+    # identical state: the FP32 worlds as the timed region left them
+    for a, b in zip(w32, w64):
+        arr, n = ctx.download(a)
+        c64.upload(b, arr, n)
+    ctx.step(w32, dt, 1)
+    c64.step(w64, dt, 1)
+    dx = dv = 0.0
+    flips = 0
+    for a, b in zip(w32, w64):
+        g, o = state_to_numpy(*ctx.download(a)), state_to_numpy(*c64.download(b))
+        dx = max(dx, float(np.max(np.abs(g["position"] - o["position"]))))
+        dv = max(dv, float(np.max(np.abs(g["velocity"] - o["velocity"]))))
+        flips += int(ctx.stats(a)["contacts"] != c64.stats(b)["contacts"])
+    for b in w64:
+        c64.destroy_world(b)
+    # drift over the whole run from the initial state, both precisions on the device
+    from cannon_physics_amd.world import Context as Ctx
+    c32 = Ctx(device=0, precision=A.PRECISION_F32)
+    d32 = [c32.create_world(recs[i]) for i in sample]
+    d64 = [c64.create_world(recs[i]) for i in sample]
+    c32.step(d32, dt, steps_total)
+    c64.step(d64, dt, steps_total)
+    drift = 0.0
+    for a, b in zip(d32, d64):
+        g, o = state_to_numpy(*c32.download(a)), state_to_numpy(*c64.download(b))
+        drift = max(drift, float(np.max(np.abs(g["position"] - o["position"]))))
+    c32.close()
+    c64.close()
+    return {"worlds": len(sample), "after_steps": preroll,
+            "one_step_max_abs_dx_m": dx, "one_step_max_abs_dv_m_per_s": dv,
+            "worlds_with_contact_count_flip": flips, "tolerance_m": 1e-4, "within_tolerance": dx <= 1e-4,
+            "drift_steps": steps_total, "drift_max_abs_dx_m": drift,
+            "reference_model": "FP64 arithmetic over FP32-stored vectors (the device's F64 mode)"}
 
 
 def main():
@@ -224,6 +287,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--preroll", type=int, default=None,
+                    help="untimed steps before the warm-up, so that the timed window sits mid-run "
+                         "(default: about half of the workload's step count)")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--worlds", type=int, default=None)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
@@ -260,6 +327,12 @@ def main():
     wids = [ctx.create_world(r) for r in recs]
     log("uploaded: %.2f GB on the device" % (ctx.bytes_in_use() / 1e9))
 
+    steps_total = {"c3": 600, "c4": 480, "c5": 300, "c1": 600, "c2": 300}[args.workload]
+    preroll = args.preroll if args.preroll is not None else max(0, steps_total // 2 - args.warmup - args.steps // 2)
+    if preroll:
+        t1 = time.perf_counter()
+        ctx.step(wids, dt, preroll)
+        log("pre-roll: %d steps in %.1fs" % (preroll, time.perf_counter() - t1))
     if args.warmup:
         # the autotuner times the level / GS workgroup sizes on the last warm-up step, once
         # contacts have formed (the choice does not change results)
@@ -320,6 +393,12 @@ def main():
                            "and a host synchronisation per step" % (len(sample), stride)}
 
     launch_shape = ctx.tuning()
+    precision = None
+    if not args.no_precision and args.precision == "f32":
+        t1 = time.perf_counter()
+        precision = precision_report(ctx, recs, wids, dyn, dt, preroll + args.warmup + 2 * K + 2, steps_total,
+                                     max(1, len(recs) // 64))
+        log("fp32 vs fp64 (%.1fs): %s" % (time.perf_counter() - t1, json.dumps(precision)))
     serving = None
     if not args.no_serving:
         ctx.close()
@@ -345,6 +424,10 @@ def main():
         "dtype": args.precision,
         "data": "synthetic",
         "config": {"workload": desc, "worlds": len(recs), "dynamic_bodies": n_dyn, "dt": dt,
+                   "run_steps": steps_total,
+                   "timed_steps": "%d..%d of the %d-step run (after %d pre-roll and %d warm-up steps)"
+                                  % (preroll + args.warmup + 1, preroll + args.warmup + K, steps_total,
+                                     preroll, args.warmup),
                    "launch_shape": launch_shape,
                    "pairs_per_step": P, "contacts_per_step": Cn, "rows_per_step": E},
         "roofline": {"bound": "hbm", "kernel": "k_gs", "achieved": achieved_gs, "peak": HBM_PEAK_GBS,
@@ -359,6 +442,7 @@ def main():
         "serving": serving,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
+        "fp32_vs_fp64": precision,
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib:

@@ -544,11 +544,13 @@ struct chp_ctx {
   Arena arena;
   std::vector<WorldSlot> worlds;
   void* d_worlds = nullptr;   // DevWorld<R>[capacity]
+  std::vector<char> h_tab;    // host image of d_worlds (the source of its last upload)
   size_t d_worlds_cap = 0;
   bool worlds_dirty = true;
   int* d_list = nullptr;
   size_t d_list_cap = 0;
   std::vector<int> h_list;  // host copy of d_list (the source of its last upload)
+  bool list_valid = false;  // d_list and the large-world tables hold h_list's tables
   std::vector<int> prev_list;   // the list and dt of the previous chp_step call
   double prev_dt = 0;
   // the large-world tables of the current list: spans, SAP tiles (int2 each), world list indices
@@ -736,6 +738,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   S.last_dt = -1;
   S.devworld.assign((char*)&D, (char*)&D + sizeof(D));
   ctx->worlds_dirty = true;
+  ctx->list_valid = false;
   return CHP_OK;
 }
 
@@ -796,6 +799,7 @@ static void release_world(chp_ctx* ctx, uint32_t id) {
   ctx->in_use -= S.bytes;
   S = WorldSlot();
   ctx->worlds_dirty = true;
+  ctx->list_valid = false;
 }
 
 static bool valid_world(chp_ctx* ctx, uint32_t id) { return ctx && id < ctx->worlds.size() && ctx->worlds[id].live; }
@@ -811,11 +815,14 @@ template <typename R> static int prepare(chp_ctx* ctx, const uint32_t* ids, uint
       HIPCHK(ctx, hipMalloc(&ctx->d_worlds, cap * sizeof(DevWorld<R>)));
       ctx->d_worlds_cap = cap;
     }
-    std::vector<DevWorld<R>> tab(ctx->worlds.size());
+    // the host image stays in the context until the next upload (the copy may read it after
+    // this call returns), as for the world list
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));   // an earlier copy may still read h_tab
+    ctx->h_tab.assign(ctx->worlds.size() * sizeof(DevWorld<R>), 0);
+    DevWorld<R>* tab = (DevWorld<R>*)ctx->h_tab.data();
     for (size_t k = 0; k < ctx->worlds.size(); k++)
       if (ctx->worlds[k].live) tab[k] = dev_of<R>(ctx->worlds[k]);
-      else memset(&tab[k], 0, sizeof(DevWorld<R>));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_worlds, tab.data(), tab.size() * sizeof(DevWorld<R>), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_worlds, tab, ctx->h_tab.size(), hipMemcpyHostToDevice, ctx->stream));
     ctx->worlds_dirty = false;
   }
   for (uint32_t k = 0; k < n; k++) {
@@ -904,6 +911,11 @@ static int ensure_queue(chp_ctx* ctx, const uint32_t* ids, uint32_t n) {
 // host copies stay alive in the context until the next upload)
 static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   const size_t n = list.size();
+  // the same list as the last upload, with no world created or destroyed since: the device
+  // tables are current
+  if (ctx->list_valid && list == ctx->h_list) return CHP_OK;
+  ctx->list_valid = false;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));   // an earlier copy may still read the host tables
   if (ctx->d_list_cap < n) {
     if (ctx->d_list) (void)hipFree(ctx->d_list);
     ctx->d_list = nullptr;
@@ -935,7 +947,10 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   ctx->h_big = spans;
   ctx->h_big.insert(ctx->h_big.end(), tiles.begin(), tiles.end());
   ctx->h_big.insert(ctx->h_big.end(), bigw.begin(), bigw.end());
-  if (ctx->h_big.empty()) return CHP_OK;
+  if (ctx->h_big.empty()) {
+    ctx->list_valid = true;
+    return CHP_OK;
+  }
   if (ctx->d_big_cap < ctx->h_big.size()) {
     if (ctx->d_big) (void)hipFree(ctx->d_big);
     ctx->d_big = nullptr;
@@ -945,6 +960,7 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   }
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_big, ctx->h_big.data(), ctx->h_big.size() * sizeof(int), hipMemcpyHostToDevice,
                              ctx->stream));
+  ctx->list_valid = true;
   return CHP_OK;
 }
 
@@ -1524,39 +1540,50 @@ template <typename R> static int serve_tick_t(chp_ctx* ctx, int max_steps, uint3
       ctx->running.pop_back();
     }
   }
-  // advance: the running requests of each dt together, by the steps the nearest finisher needs
-  int left = max_steps;
-  while (left > 0 && !ctx->running.empty()) {
-    double dt = ctx->reqs[ctx->running[0]].dt;
-    std::vector<uint64_t> group;
-    int k = left;
-    for (uint64_t t : ctx->running) {
-      const chp_ctx::Request& q = ctx->reqs[t];
-      if (q.dt != dt) continue;
-      group.push_back(t);
-      k = std::min(k, q.steps_total - q.steps_done);
-    }
-    std::vector<uint32_t> ids;
-    for (uint64_t t : group) ids.push_back(ctx->reqs[t].world);
-    std::vector<int> status;
-    int r = run_steps<R>(ctx, ids.data(), (uint32_t)ids.size(), dt, k, true, &status);
-    if (r != CHP_OK) return r;
-    for (size_t g = 0; g < group.size(); g++) {
-      chp_ctx::Request& q = ctx->reqs[group[g]];
-      if (status[g]) {
-        int err = (status[g] & WS_NONFINITE) ? CHP_E_NONFINITE : CHP_E_CAPACITY;
-        serve_retire<R>(ctx, group[g], CHP_REQ_FAILED, err,
-                        err == CHP_E_NONFINITE ? "non-finite state" : "a per-world slab overflowed");
-      } else {
-        q.steps_done += k;
-        if (q.steps_done >= q.steps_total) serve_retire<R>(ctx, group[g], CHP_REQ_DONE, CHP_OK, "");
+  // advance: every distinct dt among the running requests gets up to max_steps steps this tick
+  // (so a request whose dt differs from the oldest one's is not held back behind it); the
+  // requests of one dt run together, each launch by the steps the nearest finisher needs
+  std::vector<double> dts;
+  for (uint64_t t : ctx->running) {
+    double dt = ctx->reqs[t].dt;
+    if (std::find(dts.begin(), dts.end(), dt) == dts.end()) dts.push_back(dt);
+  }
+  for (double dt : dts) {
+    int left = max_steps;
+    while (left > 0) {
+      std::vector<uint64_t> group;
+      int k = left;
+      for (uint64_t t : ctx->running) {
+        const chp_ctx::Request& q = ctx->reqs[t];
+        if (q.dt != dt || q.state != CHP_REQ_RUNNING) continue;
+        group.push_back(t);
+        k = std::min(k, q.steps_total - q.steps_done);
       }
+      if (group.empty()) break;
+      std::vector<uint32_t> ids;
+      for (uint64_t t : group) ids.push_back(ctx->reqs[t].world);
+      std::vector<int> status;
+      int r = run_steps<R>(ctx, ids.data(), (uint32_t)ids.size(), dt, k, true, &status);
+      if (r != CHP_OK) return r;
+      for (size_t g = 0; g < group.size(); g++) {
+        chp_ctx::Request& q = ctx->reqs[group[g]];
+        if (status[g]) {
+          int err = (status[g] & WS_NONFINITE) ? CHP_E_NONFINITE : CHP_E_CAPACITY;
+          serve_retire<R>(ctx, group[g], CHP_REQ_FAILED, err,
+                          err == CHP_E_NONFINITE ? "non-finite state" : "a per-world slab overflowed");
+        } else {
+          q.steps_done += k;
+          if (q.steps_done >= q.steps_total) serve_retire<R>(ctx, group[g], CHP_REQ_DONE, CHP_OK, "");
+        }
+      }
+      left -= k;
     }
+  }
+  {
     std::vector<uint64_t> still;
     for (uint64_t t : ctx->running)
       if (ctx->reqs[t].state == CHP_REQ_RUNNING) still.push_back(t);
     ctx->running.swap(still);
-    left -= k;
   }
   if (nrun) *nrun = (uint32_t)ctx->running.size();
   if (nq) *nq = (uint32_t)ctx->queue.size();
@@ -1658,8 +1685,9 @@ int chp_world_create(chp_ctx* ctx, const chp_scene* scene, uint32_t* world_id) {
 }
 
 int chp_world_destroy(chp_ctx* ctx, uint32_t id) {
-  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!ctx) return CHP_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   WorldSlot& S = ctx->worlds[id];
@@ -1667,16 +1695,21 @@ int chp_world_destroy(chp_ctx* ctx, uint32_t id) {
   ctx->in_use -= S.bytes;
   S = WorldSlot();
   ctx->worlds_dirty = true;
+  ctx->list_valid = false;
   return CHP_OK;
 }
 
 int chp_world_num_bodies(chp_ctx* ctx, uint32_t id, uint32_t* n) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
   if (!valid_world(ctx, id) || !n) return fail(ctx, CHP_E_INVALID, "unknown world");
   *n = (uint32_t)ctx->worlds[id].nb;
   return CHP_OK;
 }
 
 int chp_world_body_info(chp_ctx* ctx, uint32_t id, chp_body_info* out, uint32_t cap) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
   if (!valid_world(ctx, id) || !out) return fail(ctx, CHP_E_INVALID, "unknown world");
   WorldSlot& S = ctx->worlds[id];
   if (cap < (uint32_t)S.nb) return fail(ctx, CHP_E_CAPACITY, "body_info buffer too small");
@@ -1685,8 +1718,9 @@ int chp_world_body_info(chp_ctx* ctx, uint32_t id, chp_body_info* out, uint32_t 
 }
 
 int chp_world_set_capacity(chp_ctx* ctx, uint32_t id, uint32_t mp, uint32_t mc, uint32_t mr) {
-  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!ctx) return CHP_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? set_capacity_t<double>(ctx, id, mp, mc, mr)
                                              : set_capacity_t<float>(ctx, id, mp, mc, mr);
@@ -1694,28 +1728,31 @@ int chp_world_set_capacity(chp_ctx* ctx, uint32_t id, uint32_t mp, uint32_t mc, 
 
 
 int chp_state_download(chp_ctx* ctx, uint32_t id, chp_body_state* out, uint32_t cap) {
-  if (!valid_world(ctx, id) || !out) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!ctx) return CHP_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id) || !out) return fail(ctx, CHP_E_INVALID, "unknown world");
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? state_io<double>(ctx, id, out, cap, true)
                                              : state_io<float>(ctx, id, out, cap, true);
 }
 
 int chp_state_upload(chp_ctx* ctx, uint32_t id, const chp_body_state* in, uint32_t n) {
-  if (!valid_world(ctx, id) || !in) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!ctx) return CHP_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id) || !in) return fail(ctx, CHP_E_INVALID, "unknown world");
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? state_io<double>(ctx, id, (chp_body_state*)in, n, false)
                                              : state_io<float>(ctx, id, (chp_body_state*)in, n, false);
 }
 
 static int step_common(chp_ctx* ctx, const uint32_t* worlds, uint32_t n, double dt, int32_t nsteps, bool wait) {
-  if (!ctx || (!worlds && n)) return fail(ctx, CHP_E_INVALID, "null argument");
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!worlds && n) return fail(ctx, CHP_E_INVALID, "null argument");
   if (!(dt > 0) || nsteps < 0) return fail(ctx, CHP_E_INVALID, "dt must be > 0 and nsteps >= 0");
   for (uint32_t k = 0; k < n; k++)
     if (!valid_world(ctx, worlds[k])) return fail(ctx, CHP_E_INVALID, "unknown world id " + std::to_string(worlds[k]));
   if (n == 0 || nsteps == 0) return CHP_OK;
-  std::lock_guard<std::mutex> g(ctx->mu);
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? step_t<double>(ctx, worlds, n, dt, nsteps, wait)
                                              : step_t<float>(ctx, worlds, n, dt, nsteps, wait);
@@ -1735,17 +1772,19 @@ int chp_sync(chp_ctx* ctx) {
 }
 
 int chp_step_plain(chp_ctx* ctx, uint32_t id, double dt, int32_t nsteps) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
   if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
   if (!(dt > 0) || nsteps < 0) return fail(ctx, CHP_E_INVALID, "dt must be > 0 and nsteps >= 0");
-  std::lock_guard<std::mutex> g(ctx->mu);
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? step_plain_t<double>(ctx, id, dt, nsteps)
                                              : step_plain_t<float>(ctx, id, dt, nsteps);
 }
 
 int chp_world_stats(chp_ctx* ctx, uint32_t id, chp_step_stats* out) {
-  if (!valid_world(ctx, id) || !out) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!ctx) return CHP_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id) || !out) return fail(ctx, CHP_E_INVALID, "unknown world");
   (void)hipSetDevice(ctx->device);
   std::vector<int> st;
   int r = read_status(ctx, &id, 1, st);
@@ -1765,8 +1804,9 @@ int chp_world_stats(chp_ctx* ctx, uint32_t id, chp_step_stats* out) {
 
 
 int chp_broadphase(chp_ctx* ctx, uint32_t id, int32_t* pi, int32_t* pj, uint32_t cap, uint32_t* np) {
-  if (!valid_world(ctx, id) || !pi || !pj || !np) return fail(ctx, CHP_E_INVALID, "bad argument");
+  if (!ctx) return CHP_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id) || !pi || !pj || !np) return fail(ctx, CHP_E_INVALID, "bad argument");
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? broadphase_t<double>(ctx, id, pi, pj, cap, np)
                                              : broadphase_t<float>(ctx, id, pi, pj, cap, np);
@@ -1775,10 +1815,11 @@ int chp_broadphase(chp_ctx* ctx, uint32_t id, int32_t* pi, int32_t* pj, uint32_t
 
 int chp_step_capture(chp_ctx* ctx, uint32_t id, double dt, int32_t* pi, int32_t* pj, uint32_t pair_cap, uint32_t* npairs,
                      chp_row* rows, uint32_t row_cap, uint32_t* nrows, int32_t* iterations) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
   if (!valid_world(ctx, id) || !pi || !pj || !npairs || !rows || !nrows || !iterations)
     return fail(ctx, CHP_E_INVALID, "bad argument");
   if (!(dt > 0)) return fail(ctx, CHP_E_INVALID, "dt must be > 0");
-  std::lock_guard<std::mutex> g(ctx->mu);
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64
              ? capture_t<double>(ctx, id, dt, pi, pj, pair_cap, npairs, rows, row_cap, nrows, iterations)
@@ -1787,10 +1828,11 @@ int chp_step_capture(chp_ctx* ctx, uint32_t id, double dt, int32_t* pi, int32_t*
 
 int chp_narrowphase(chp_ctx* ctx, uint32_t id, const int32_t* pi, const int32_t* pj, uint32_t npairs, double dt,
                     chp_equation* out, uint32_t cap, uint32_t* neq, uint32_t* nf, uint32_t* nc) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
   if (!valid_world(ctx, id) || (npairs && (!pi || !pj)) || !neq || !nf || !nc || (cap && !out))
     return fail(ctx, CHP_E_INVALID, "bad argument");
   if (!(dt > 0)) return fail(ctx, CHP_E_INVALID, "dt must be > 0");
-  std::lock_guard<std::mutex> g(ctx->mu);
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? narrowphase_t<double>(ctx, id, pi, pj, npairs, dt, out, cap, neq, nf, nc)
                                              : narrowphase_t<float>(ctx, id, pi, pj, npairs, dt, out, cap, neq, nf, nc);
@@ -1798,9 +1840,10 @@ int chp_narrowphase(chp_ctx* ctx, uint32_t id, const int32_t* pi, const int32_t*
 
 int chp_solve(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t n, double dt, int32_t* iterations,
               double* multipliers) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
   if (!valid_world(ctx, id) || (n && !eqs) || !iterations) return fail(ctx, CHP_E_INVALID, "bad argument");
   if (!(dt > 0)) return fail(ctx, CHP_E_INVALID, "dt must be > 0");
-  std::lock_guard<std::mutex> g(ctx->mu);
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? solve_t<double>(ctx, id, eqs, n, dt, iterations, multipliers)
                                              : solve_t<float>(ctx, id, eqs, n, dt, iterations, multipliers);

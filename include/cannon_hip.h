@@ -315,10 +315,11 @@ int chp_solve(chp_ctx* ctx, uint32_t world_id, const chp_equation* eqs, uint32_t
 /* ---- serving: continuous batching of requests --------------------------------------------- */
 /* A request is a world and a number of steps.  chp_serve_submit replays its constructors on the
  * host and queues it; chp_serve_tick admits queued requests in arrival order while the
- * context's HBM budget allows, advances every admitted request (one launch per stage over all
- * of them, up to max_steps steps), retires the finished ones (their final state is kept on the
- * host and their device memory freed) and returns.  Results do not depend on what else is in
- * the batch. */
+ * context's HBM budget allows, advances every admitted request by up to max_steps steps (the
+ * requests sharing a dt in one launch per stage; each distinct dt gets its own max_steps, so no
+ * request waits behind requests of another dt), retires the finished ones (their final state is
+ * kept on the host and their device memory freed) and returns.  Results do not depend on what
+ * else is in the batch. */
 enum { CHP_REQ_QUEUED = 0, CHP_REQ_RUNNING = 1, CHP_REQ_DONE = 2, CHP_REQ_FAILED = 3, CHP_REQ_CANCELLED = 4 };
 typedef struct chp_request_status {
   int32_t state;                         /* CHP_REQ_* */

@@ -127,6 +127,7 @@ int orc_broadphase(void* wp, int* pi, int* pj, int cap) {
 }  /* extern "C" */
 
 namespace orc {
+int sat_probe(World& w, int a, int b, bool face0, V3& axis, R& depth);
 void narrowphase_equations(World& w, const std::vector<int>& pi, const std::vector<int>& pj, R dt,
                            std::vector<Equation>& out, int& nf, int& nc);
 int solve_equations(World& w, std::vector<Equation>& eqs, R dt, std::vector<R>& multipliers);
@@ -153,6 +154,17 @@ int orc_narrowphase(void* wp, const int* pi, const int* pj, int n, double dt, ch
     o.restitution = (double)e.restitution;
   }
   return (int)eqs.size();
+}
+
+/* findSeparatingAxis between the first shapes of bodies a and b (see sat_probe) */
+int orc_sat(void* wp, int a, int b, int face0_only, float* axis, double* depth) {
+  World& w = *(World*)wp;
+  V3 ax;
+  R d = 0;
+  int r = orc::sat_probe(w, a, b, face0_only != 0, ax, d);
+  put3(axis, ax);
+  *depth = (double)d;
+  return r;
 }
 
 /* Solver slot: GSSolver.solve on the current state for the given equations */

@@ -485,7 +485,8 @@ struct NP {
   }
   /* findSeparatingAxis, convex_polyhedron.dart:232-356 */
   static bool findSeparatingAxis(const Hull& A, const Hull& B, const V3& posA, const Q4& quatA,
-                                 const V3& posB, const Q4& quatB, V3& target, const std::vector<int>* faceListA) {
+                                 const V3& posB, const Q4& quatB, V3& target, const std::vector<int>* faceListA,
+                                 R* depthOut = nullptr) {
     R dmin = INF;
     if (A.hasAxes) {
       size_t nA = faceListA ? faceListA->size() : A.faces.size();
@@ -524,6 +525,7 @@ struct NP {
     V3 deltaC;
     sub2(posB, posA, deltaC);
     if (dot(deltaC, target) > (R)0) negate(target);
+    if (depthOut) *depthOut = dmin;
     return true;
   }
   /* clipFaceAgainstPlane, convex_polyhedron.dart:545-587 */
@@ -1193,6 +1195,34 @@ void narrowphase_equations(World& w, const std::vector<int>& pi, const std::vect
     }
     if (keep(c)) { out.push_back(c); nc++; }
   }
+}
+
+/* Probe of findSeparatingAxis (convex_polyhedron.dart:232-356) between the first shapes of two
+ * bodies (box representation or convex hull) at their current world poses, as convexConvex
+ * calls it; for the tests that pin the axis search.  Returns 1 with the oriented axis and the
+ * least overlap depth when no separating axis was found, 0 when separated, -1 when a shape is
+ * not convex. */
+int sat_probe(World& w, int a, int b, bool face0, V3& axis, R& depth) {
+  Body* ba = w.bodies[a];
+  Body* bb = w.bodies[b];
+  auto hull_of = [](Shape* s) -> const Hull* {
+    if (s->type == CHP_SHAPE_BOX) return &s->repr;
+    if (s->type == CHP_SHAPE_CONVEX) return s->hull;
+    return nullptr;
+  };
+  const Hull* A = hull_of(ba->shapes[0]);
+  const Hull* B = hull_of(bb->shapes[0]);
+  if (!A || !B) return -1;
+  V3 xi, xj;
+  Q4 qi, qj;
+  qmultiply2(ba->quaternion, ba->shapeOrientations[0], qi);
+  vmult(ba->quaternion, ba->shapeOffsets[0], xi);
+  add2(xi, ba->position, xi);
+  qmultiply2(bb->quaternion, bb->shapeOrientations[0], qj);
+  vmult(bb->quaternion, bb->shapeOffsets[0], xj);
+  add2(xj, bb->position, xj);
+  std::vector<int> f0 = {0};
+  return NP::findSeparatingAxis(*A, *B, xi, qi, xj, qj, axis, face0 ? &f0 : nullptr, &depth) ? 1 : 0;
 }
 
 /* The Solver slot: GSSolver.solve(dt, world) on the world's current state for given equations

@@ -45,6 +45,7 @@ def lib(mode="f64"):
         "orc_narrowphase": (C.c_int, [vp, P(C.c_int), P(C.c_int), C.c_int, C.c_double, P(A.Equation), C.c_int,
                                       P(C.c_int), P(C.c_int)]),
         "orc_solve": (C.c_int, [vp, P(A.Equation), C.c_int, C.c_double, P(C.c_double)]),
+        "orc_sat": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -135,6 +136,26 @@ class OracleWorld:
         nf, nc = C.c_int(), C.c_int()
         k = self.L.orc_narrowphase(self.ptr, api, apj, n, dt, out, cap, C.byref(nf), C.byref(nc))
         return out, k, nf.value, nc.value
+
+    def sat(self, a, b, face0_only=False):
+        """findSeparatingAxis between the first shapes of bodies a and b at their current poses:
+        None when separated, else (oriented axis, least overlap depth)"""
+        ax = (C.c_float * 3)()
+        d = C.c_double()
+        r = self.L.orc_sat(self.ptr, a, b, int(face0_only), ax, C.byref(d))
+        if r < 0:
+            raise ValueError("not a convex pair")
+        return (np.array(ax[:], dtype=np.float64), d.value) if r == 1 else None
+
+    def contact_equations(self, pi, pj, dt=1 / 60):
+        """the contact equations (not the friction ones) the Narrowphase slot makes for the
+        given pairs: dicts with body_i, body_j, ri, rj, n"""
+        out, k, nf, nc = self.narrowphase(pi, pj, dt)
+        res = []
+        for e in list(out)[nf:k]:
+            res.append({"body_i": e.body_i, "body_j": e.body_j, "ri": np.array(e.ri[:], dtype=np.float64),
+                        "rj": np.array(e.rj[:], dtype=np.float64), "n": np.array(e.n[:], dtype=np.float64)})
+        return res
 
     def solve(self, eqs, n, dt):
         """Solver slot on the current state: (iterations, multipliers)"""
