@@ -237,8 +237,9 @@ def precision_report(ctx, recs, wids, dyn, dt, preroll, steps_total, stride):
     """FP32 device path against the reference's own arithmetic (FP64 expressions over FP32
     storage: the device's F64 mode, which the parity tests hold bit-identical to the oracle's
     FP64 mode) on every stride-th world of the batch:
-      one step from identical state -- max |dx|, max |dv|, and the worlds whose contact count
-      differs (a narrowphase decision that flipped);
+      one step from identical state -- max |dx|, max |dv| over the worlds where both make the
+      same narrowphase decisions (the same contact count); the worlds whose contact count
+      differs (a decision that flipped) and their max |dx|, reported apart;
       drift -- both paths from the initial state over the config's step count, max |dx|."""
     import numpy as np
     from cannon_physics_amd import _abi as A
@@ -253,13 +254,17 @@ def precision_report(ctx, recs, wids, dyn, dt, preroll, steps_total, stride):
         c64.upload(b, arr, n)
     ctx.step(w32, dt, 1)
     c64.step(w64, dt, 1)
-    dx = dv = 0.0
+    dx = dv = dx_flip = 0.0
     flips = 0
     for a, b in zip(w32, w64):
         g, o = state_to_numpy(*ctx.download(a)), state_to_numpy(*c64.download(b))
-        dx = max(dx, float(np.max(np.abs(g["position"] - o["position"]))))
-        dv = max(dv, float(np.max(np.abs(g["velocity"] - o["velocity"]))))
-        flips += int(ctx.stats(a)["contacts"] != c64.stats(b)["contacts"])
+        ex = float(np.max(np.abs(g["position"] - o["position"])))
+        ev = float(np.max(np.abs(g["velocity"] - o["velocity"])))
+        if ctx.stats(a)["contacts"] != c64.stats(b)["contacts"]:
+            flips += 1
+            dx_flip = max(dx_flip, ex)
+        else:
+            dx, dv = max(dx, ex), max(dv, ev)
     for b in w64:
         c64.destroy_world(b)
     # drift over the whole run from the initial state, both precisions on the device
@@ -277,7 +282,8 @@ def precision_report(ctx, recs, wids, dyn, dt, preroll, steps_total, stride):
     c64.close()
     return {"worlds": len(sample), "after_steps": preroll,
             "one_step_max_abs_dx_m": dx, "one_step_max_abs_dv_m_per_s": dv,
-            "worlds_with_contact_count_flip": flips, "tolerance_m": 1e-4, "within_tolerance": dx <= 1e-4,
+            "worlds_with_contact_count_flip": flips, "flip_worlds_max_abs_dx_m": dx_flip,
+            "tolerance_m": 1e-4, "within_tolerance": dx <= 1e-4,
             "drift_steps": steps_total, "drift_max_abs_dx_m": drift,
             "reference_model": "FP64 arithmetic over FP32-stored vectors (the device's F64 mode)"}
 

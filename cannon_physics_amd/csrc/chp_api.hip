@@ -230,17 +230,23 @@ __global__ __launch_bounds__(64) void k_narrow(const DevWorld<R>* __restrict__ w
   if (W.nb >= BP_BIG) return;  // k_big_nplace
   st_narrow_place(W, list[blockIdx.x], qcls + blockIdx.x, n, q, sh);
 }
+// classes whose resolvers clip polygons (convex-convex, and the mixed class) hold the clip
+// polygons of the block's 64 lanes in LDS
+constexpr bool cls_clips(int c) { return c == NP_CC || c == NP_OTHER; }
 template <typename R, int CLS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_narrow_cls(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
                                                    const int* __restrict__ bounds, double dt) {
-  st_narrow_class<R, CLS>(worlds, q, bounds, (R)dt);
+  __shared__ f3 cb[cls_clips(CLS) ? 2 * CLIP_THREAD * 64 : 1];
+  st_narrow_class<R, CLS>(worlds, q, bounds, (R)dt, cb);
 }
 template <typename R, int CLS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_narrow_wave(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
                                                      const int* __restrict__ bounds, double dt) {
-  __shared__ f3 ws[4][NPCtx<R, true>::WS_F3];   // one LDS workspace per wave
+  __shared__ f3 ws[4][NPCtx<R, true>::WS_F3];            // one LDS workspace per wave
+  __shared__ ContactRec<R> loc[4][CHP_KMAX];              // and one contact buffer
   constexpr unsigned kinds = CLS == NP_WHC ? (NK_CC | NK_HC) : NK_ALL;
-  st_narrow_wave<R, kinds>(worlds, q + bounds[CLS], bounds[CLS + 1] - bounds[CLS], (R)dt, ws[threadIdx.x >> 6]);
+  st_narrow_wave<R, kinds>(worlds, q + bounds[CLS], bounds[CLS + 1] - bounds[CLS], (R)dt, ws[threadIdx.x >> 6],
+                           loc[threadIdx.x >> 6]);
 }
 
 
@@ -411,8 +417,9 @@ template <typename R>
 __global__ __launch_bounds__(64) void k_np_slot(const DevWorld<R>* __restrict__ worlds, int id, double dt,
                                                 chp_equation* __restrict__ out, int cap) {
   __shared__ int sh[64];
+  __shared__ f3 cb[2 * CLIP_THREAD * 64];
   const DevWorld<R>& W = worlds[id];
-  st_narrow(W, (R)dt);
+  st_narrow(W, (R)dt, cb);
   st_equations(W, (R)dt, out, cap, sh);
 }
 
@@ -684,7 +691,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   ARR(si_group, H.si_group); ARR(si_mask, H.si_mask); ARR(si_material, H.si_material);
   ARR(si_off, H.si_off); ARR(si_ori, H.si_ori); ARR(si_half, H.si_half); ARR(si_radius, H.si_radius); ARR(si_bsr, H.si_bsr);
   ARR(h_vstart, H.H.vstart); ARR(h_vcount, H.H.vcount); ARR(h_fstart, H.H.fstart); ARR(h_fcount, H.H.fcount);
-  ARR(h_estart, H.H.estart); ARR(h_ecount, H.H.ecount); ARR(h_axes, H.H.axes);
+  ARR(h_estart, H.H.estart); ARR(h_ecount, H.H.ecount); ARR(h_axes, H.H.axes); ARR(h_maxface, H.H.maxface);
   std::vector<R> hbsr(H.H.bsr.begin(), H.H.bsr.end());
   ARR(h_bsr, hbsr);
   ARR(h_verts, H.H.verts); ARR(h_fnormal, H.H.fnormal); ARR(h_edges, H.H.edges);
@@ -874,6 +881,11 @@ static int check_status(chp_ctx* ctx, const uint32_t* ids, uint32_t n) {
       snprintf(buf, sizeof buf, "world %u produced non-finite state and is quarantined", ids[k]);
       return fail(ctx, CHP_E_NONFINITE, buf);
     }
+    if (s & WS_STAGING) {
+      snprintf(buf, sizeof buf, "world %u: a contact polygon outgrew its clipping buffer (%d points)", ids[k],
+               CLIP_WAVE);
+      return fail(ctx, CHP_E_CAPACITY, buf);
+    }
     snprintf(buf, sizeof buf,
              "world %u exceeded a slab (status %d): pairs %d/%d, contacts %d/%d, rows %d/%d; raise it with "
              "chp_world_set_capacity",
@@ -964,29 +976,49 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   return CHP_OK;
 }
 
-static const int2* big_spans(const chp_ctx* ctx) { return (const int2*)ctx->d_big; }
-static const int* big_worlds(const chp_ctx* ctx) {
-  return (const int*)((const int2*)ctx->d_big + ctx->big_nspans + ctx->big_ntiles);
+// The device buffers the kernels of one step run over: the DevWorld table, the world list, the
+// large-world tables (spans, SAP tiles, list indices of the large worlds) and the narrowphase
+// queue with its class counts.  The context's own buffers fill it for chp_step; the
+// allocation-free entry point fills it from its fixed scratch array.
+struct StepView {
+  const void* worlds = nullptr;
+  const int* list = nullptr;
+  const int* big = nullptr;
+  int nspans = 0, ntiles = 0, nw = 0, rounds = 0, n_sap = 0;
+  int2* q = nullptr;
+  size_t qcap = 0;
+  int* qcls = nullptr;
+  const int2* spans() const { return (const int2*)big; }
+  const int2* tiles() const { return (const int2*)big + nspans; }
+  const int* bigw() const { return (const int*)((const int2*)big + nspans + ntiles); }
+};
+static StepView ctx_view(const chp_ctx* ctx) {
+  StepView v;
+  v.worlds = ctx->d_worlds;
+  v.list = ctx->d_list;
+  v.big = ctx->d_big;
+  v.nspans = ctx->big_nspans; v.ntiles = ctx->big_ntiles; v.nw = ctx->big_nw; v.rounds = ctx->big_rounds;
+  v.n_sap = ctx->n_sap;
+  v.q = ctx->d_q;
+  v.qcap = ctx->d_q_cap;
+  v.qcls = ctx->d_qcls;
+  return v;
 }
 
-// the broadphase of the large worlds of the current list (after k_bp, which leaves them alone)
-template <typename R> static void launch_big_bp(chp_ctx* ctx, const int* d_list) {
-  if (ctx->big_nw == 0) return;
-  const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
-  hipStream_t st = ctx->stream;
-  const int2* spans = (const int2*)ctx->d_big;
-  const int2* tiles = spans + ctx->big_nspans;
-  const int* bigw = (const int*)(tiles + ctx->big_ntiles);
-  const int ns = ctx->big_nspans, rounds = ctx->big_rounds;
-  k_big_prestep<R><<<ns, 64, 0, st>>>(dw, d_list, spans);
-  if (ctx->big_ntiles) {
-    k_sap_tile<R><<<ctx->big_ntiles, 256, 0, st>>>(dw, d_list, tiles, rounds == 0);
+// the broadphase of the large worlds of the list (after k_bp, which leaves them alone)
+template <typename R> static void launch_big_bp(const StepView& V, hipStream_t st) {
+  if (V.nw == 0) return;
+  const DevWorld<R>* dw = (const DevWorld<R>*)V.worlds;
+  const int ns = V.nspans, rounds = V.rounds;
+  k_big_prestep<R><<<ns, 64, 0, st>>>(dw, V.list, V.spans());
+  if (V.ntiles) {
+    k_sap_tile<R><<<V.ntiles, 256, 0, st>>>(dw, V.list, V.tiles(), rounds == 0);
     for (int r = 0; r < rounds; r++)
-      k_sap_merge<R><<<ns, 64, 0, st>>>(dw, d_list, spans, SAP_TILE << r, r & 1, r == rounds - 1);
+      k_sap_merge<R><<<ns, 64, 0, st>>>(dw, V.list, V.spans(), SAP_TILE << r, r & 1, r == rounds - 1);
   }
-  k_big_rows<R><<<4 * ns, 256, 0, st>>>(dw, d_list, spans, 0);
-  k_big_scan<R><<<ctx->big_nw, 1024, 0, st>>>(dw, d_list, bigw);
-  k_big_rows<R><<<4 * ns, 256, 0, st>>>(dw, d_list, spans, 1);
+  k_big_rows<R><<<4 * ns, 256, 0, st>>>(dw, V.list, V.spans(), 0);
+  k_big_scan<R><<<V.nw, 1024, 0, st>>>(dw, V.list, V.bigw());
+  k_big_rows<R><<<4 * ns, 256, 0, st>>>(dw, V.list, V.spans(), 1);
 }
 
 enum { K_BP = 0, K_NARROW, K_ROWS, K_LEVELS, K_GSROWS, K_GS, K_POST, K_NSTAGE };
@@ -999,9 +1031,10 @@ static const int kTuneBlocks[3] = {64, 128, 256};
 // the stage kernels of one step.  tune: the level and GS stages run once per candidate
 // workgroup size, timed by ctx->tev (they rebuild the same schedule from the same rows, so every
 // run leaves the same result)
-template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, int n, int nbig, int max_nb, double dt,
+template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, int n, int nbig, int max_nb, double dt,
                                              hipEvent_t* ev, bool tune = false) {
-  const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
+  const DevWorld<R>* dw = (const DevWorld<R>*)V.worlds;
+  const int* d_list = V.list;
   auto groups = [&](int bd) { return nbig + (n - nbig + bd / 64 - 1) / (bd / 64); };
   const int bcap = big_lds_cap(nbig, max_nb, ctx->max_lds);
   hipStream_t st = ctx->stream;
@@ -1011,47 +1044,46 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     int lds_nb = max_nb;
     while (lds_nb > 0 && BpLds<R>::bytes(lds_nb) > ctx->max_lds - 4096) lds_nb--;
     k_bp<R, false><<<n, 256, BpLds<R>::bytes(lds_nb), st>>>(dw, d_list, lds_nb);
-    if (ctx->n_sap) k_bp<R, true><<<n, 256, 0, st>>>(dw, d_list, 0);
-    launch_big_bp<R>(ctx, d_list);
+    if (V.n_sap) k_bp<R, true><<<n, 256, 0, st>>>(dw, d_list, 0);
+    launch_big_bp<R>(V, st);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
   {
-    const int qcap = (int)ctx->d_q_cap;
-    int* bounds = ctx->d_qcls + (size_t)n * NP_NCLS;
-    k_narrow_count<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls, n);
-    if (ctx->big_nw) k_big_ncount<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls, n);
-    k_narrow_scan<<<1, 1024, 0, st>>>(ctx->d_qcls, n, bounds);
-    k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, ctx->d_qcls, ctx->d_q, n);
-    if (ctx->big_nw)
-      k_big_nplace<R><<<ctx->big_nspans, 256, 0, st>>>(dw, d_list, big_spans(ctx), ctx->d_qcls, ctx->d_q, n);
+    const int qcap = (int)V.qcap;
+    int* bounds = V.qcls + (size_t)n * NP_NCLS;
+    k_narrow_count<R><<<n, 64, 0, st>>>(dw, d_list, V.qcls, n);
+    if (V.nw) k_big_ncount<R><<<V.nspans, 256, 0, st>>>(dw, d_list, V.spans(), V.qcls, n);
+    k_narrow_scan<<<1, 1024, 0, st>>>(V.qcls, n, bounds);
+    k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, V.qcls, V.q, n);
+    if (V.nw) k_big_nplace<R><<<V.nspans, 256, 0, st>>>(dw, d_list, V.spans(), V.qcls, V.q, n);
     // one kernel per thread class (each compiles only its resolver), up to 4 waves per SIMD
     const int blocks = (std::max(64, std::min(qcap, 256 * 4 * 4 * 64)) + 63) / 64;
-    k_narrow_cls<R, NP_SS><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
-    k_narrow_cls<R, NP_SP><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
-    k_narrow_cls<R, NP_SB><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
-    k_narrow_cls<R, NP_SC><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
-    k_narrow_cls<R, NP_SH><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
-    k_narrow_cls<R, NP_PC><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
-    k_narrow_cls<R, NP_CC><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
-    k_narrow_cls<R, NP_OTHER><<<blocks, 64, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_cls<R, NP_SS><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
+    k_narrow_cls<R, NP_SP><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
+    k_narrow_cls<R, NP_SB><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
+    k_narrow_cls<R, NP_SC><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
+    k_narrow_cls<R, NP_SH><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
+    k_narrow_cls<R, NP_PC><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
+    k_narrow_cls<R, NP_CC><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
+    k_narrow_cls<R, NP_OTHER><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
     const int waves = std::max(1, std::min(qcap, 4096));
-    k_narrow_wave<R, NP_WHC><<<(waves + 3) / 4, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_wave<R, NP_WHC><<<(waves + 3) / 4, 256, 0, st>>>(dw, V.q, bounds, dt);
     // compound-body pairs are rare: a small grid, strided over however many there are
-    k_narrow_wave<R, NP_WAVE><<<32, 256, 0, st>>>(dw, ctx->d_q, bounds, dt);
+    k_narrow_wave<R, NP_WAVE><<<32, 256, 0, st>>>(dw, V.q, bounds, dt);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
   // rows: offsets, then the rows' body pairs; the level schedule; then the rows, written as GS
   // records in level order (keep: also as row records, for the capture)
   const int keep = ctx->keep_rows ? 1 : 0;
-  const int* qbounds = ctx->d_qcls + (size_t)n * NP_NCLS;
-  const int qblocks = (std::max(64, std::min((int)ctx->d_q_cap, 256 * 4 * 4 * 64)) + 63) / 64;
+  const int* qbounds = V.qcls + (size_t)n * NP_NCLS;
+  const int qblocks = (std::max(64, std::min((int)V.qcap, 256 * 4 * 4 * 64)) + 63) / 64;
   auto build_rows = [&]() {
-    k_pair_rows<R, ROWS_GS><<<qblocks, 64, 0, st>>>(dw, ctx->d_q, qbounds, dt, keep);
+    k_pair_rows<R, ROWS_GS><<<qblocks, 64, 0, st>>>(dw, V.q, qbounds, dt, keep);
     k_joint_rows<R, ROWS_GS><<<n, 64, 0, st>>>(dw, d_list, dt, keep);
   };
   k_rows<R><<<n, 128, 0, st>>>(dw, d_list);
-  if (ctx->big_nw) k_big_rowscan<R><<<ctx->big_nw, 1024, 0, st>>>(dw, d_list, big_worlds(ctx));
-  k_pair_rows<R, ROWS_AB><<<qblocks, 64, 0, st>>>(dw, ctx->d_q, qbounds, dt, 0);
+  if (V.nw) k_big_rowscan<R><<<V.nw, 1024, 0, st>>>(dw, d_list, V.bigw());
+  k_pair_rows<R, ROWS_AB><<<qblocks, 64, 0, st>>>(dw, V.q, qbounds, dt, 0);
   k_joint_rows<R, ROWS_AB><<<n, 64, 0, st>>>(dw, d_list, dt, 0);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
   if (tune) {
@@ -1080,7 +1112,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const int* d_list, in
     k_gs<R, true><<<groups(bg), bg, 0, st>>>(dw, d_list, nbig, n, bcap, keep);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
   }
-  if (ctx->big_nw) k_big_post<R><<<ctx->big_nspans, 64, 0, st>>>(dw, d_list, big_spans(ctx), dt);
+  if (V.nw) k_big_post<R><<<V.nspans, 64, 0, st>>>(dw, d_list, V.spans(), dt);
   k_post<R><<<n, 256, 0, st>>>(dw, d_list, dt);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[7], st));
   HIPCHK(ctx, hipGetLastError());
@@ -1107,7 +1139,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
     if (!may_capture) return CHP_OK;
     drop_graph(ctx);
     HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-    int r = launch_step<R>(ctx, ctx->d_list, (int)list.size(), nbig, max_nb, dt, nullptr);
+    int r = launch_step<R>(ctx, ctx_view(ctx), (int)list.size(), nbig, max_nb, dt, nullptr);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (r != CHP_OK) { if (graph) (void)hipGraphDestroy(graph); return r; }
@@ -1174,7 +1206,7 @@ template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, ui
       HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
       continue;
     }
-    r = launch_step<R>(ctx, ctx->d_list, (int)n, nbig, max_nb, dt,
+    r = launch_step<R>(ctx, ctx_view(ctx), (int)n, nbig, max_nb, dt,
                        ctx->profile ? &ctx->sev[s * (K_NSTAGE + 1)] : nullptr, tune && s == 0);
     if (r != CHP_OK) return r;
   }
@@ -1226,7 +1258,7 @@ template <typename R> static int step_plain_t(chp_ctx* ctx, uint32_t id, double 
   if (r != CHP_OK) return r;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int s = 0; s < nsteps; s++) {
-    r = launch_step<R>(ctx, ctx->d_list, 1, ctx->worlds[id].nb > SMALL_NB ? 1 : 0, ctx->worlds[id].nb, dt, nullptr);
+    r = launch_step<R>(ctx, ctx_view(ctx), 1, ctx->worlds[id].nb > SMALL_NB ? 1 : 0, ctx->worlds[id].nb, dt, nullptr);
     if (r != CHP_OK) return r;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the slot-by-slot drop-in hands back each step
   }
@@ -1318,7 +1350,7 @@ template <typename R> static int broadphase_t(chp_ctx* ctx, uint32_t id, int32_t
   if (ctx->worlds[id].nb >= BP_BIG) {
     r = upload_list(ctx, {(int)id});
     if (r != CHP_OK) return r;
-    launch_big_bp<R>(ctx, ctx->d_list);
+    launch_big_bp<R>(ctx_view(ctx), ctx->stream);
   } else {
     k_broadphase<R><<<1, CHP_BLOCK, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, (int)id);
   }

@@ -20,7 +20,7 @@
 namespace chp {
 
 struct HullBuf {  // hull table under construction
-  std::vector<int> vstart, vcount, fstart, fcount, estart, ecount, axes;
+  std::vector<int> vstart, vcount, fstart, fcount, estart, ecount, axes, maxface;
   std::vector<double> bsr;
   std::vector<f3> verts, fnormal, edges;
   std::vector<int> f_istart{0}, f_idx;
@@ -39,6 +39,9 @@ int hull_add(HullBuf& H, const std::vector<f3>& verts, const std::vector<std::ve
   H.fstart.push_back((int)H.fnormal.size());
   H.fcount.push_back((int)faces.size());
   H.axes.push_back(axes ? 1 : 0);
+  int mf = 0;
+  for (const auto& f : faces) mf = std::max(mf, (int)f.size());
+  H.maxface.push_back(mf);
   for (const f3& v : verts) H.verts.push_back(v);
   for (const auto& f : faces) {
     for (int k : f) H.f_idx.push_back(k);
@@ -189,6 +192,10 @@ template <typename R> struct Builder {
           face.push_back((int)S.hull_face_vertex[t]);
         }
         if (face.size() < 3) fail("hull face with fewer than 3 vertices");
+        if (face.size() > (size_t)MAX_FACE_VERTS)
+          fail("hull face with " + std::to_string(face.size()) + " vertices: faces are limited to " +
+               std::to_string(MAX_FACE_VERTS) + " vertices (a contact polygon is clipped in a buffer of " +
+               std::to_string(CLIP_WAVE) + " points); split the face");
         f.push_back(face);
       }
       if (v.empty() || f.empty()) fail("empty convex hull");

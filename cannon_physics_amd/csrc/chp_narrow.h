@@ -8,8 +8,6 @@
 
 namespace chp {
 
-#define CHP_MAXPOLY 16
-
 template <typename R> struct HullView {
   const f3* v; int nv;
   const f3* fn; const int* fidx; const int* fis; int nf;  // face f: fidx[fis[f] .. fis[f+1])
@@ -43,13 +41,39 @@ template <typename R> CHP_HD HullView<R> hull_view(const DevWorld<R>& W, int h) 
 struct SRef { int resp; int mat; };   // Shape.collisionResponse / Shape.material
 
 // where resolvers put contacts.  WAVE: every lane of a wave computes the same contacts and
-// lane 0 stores them.
+// lane 0 stores them.  Two kinds:
+//  - buffered: into buf (a local or LDS array of cap records); writes beyond cap are counted,
+//    not stored, and the caller runs the pair again straight into the slab;
+//  - direct (slot != nullptr): a resolver that emits all of a pair's contacts in one go first
+//    calls begin(k), which reserves k consecutive records of the world's staging slab through
+//    its counter; push then writes there.  Counts of enabled contacts and friction rows are kept
+//    as the records go out.
 template <typename R, bool WAVE = false> struct Sink {
-  ContactRec<R>* buf;  // local array or global slab
-  int cap;             // writes beyond cap are counted, not stored
+  ContactRec<R>* buf;  // buffered: local or LDS array; direct: the world's contact slab
+  int cap;             // buffered: records buf holds; direct: the slab's capacity
   int n;
   int overflow;
+  int* slot = nullptr;     // direct: the world's staging counter
+  int base = 0;            // direct: first reserved record
+  int nc = 0, nf = 0;      // enabled contacts, friction rows (direct)
+  bool begun = false;
+  CHP_HD void begin(int k) {
+    if (!slot) return;
+    if (begun) { overflow = 1; return; }   // a direct sink takes one resolver call per pair
+    begun = true;
+    if (k <= 0) return;
+    base = atomicAdd(slot, k);
+    if (base + k > cap) overflow = 1;       // the slab is full: nothing is written
+  }
   CHP_HD void push(const ContactRec<R>& c) {
+    if (slot) {
+      if (!overflow && begun) {
+        buf[base + n] = c;
+        if (c.flags & 1) { nc++; if (c.flags & 2) nf += 2; }
+      }
+      n++;
+      return;
+    }
     if (n < cap) {
       if (!WAVE || (threadIdx.x & 63) == 0) buf[n] = c;
     } else {
@@ -57,6 +81,14 @@ template <typename R, bool WAVE = false> struct Sink {
     }
     n++;
   }
+};
+
+// a polygon in LDS or a local array: element k at p[k * s] (lanes of a wave interleave their
+// polygons with s = 64, so that neighbouring lanes touch neighbouring words)
+struct Poly {
+  f3* p;
+  int s;
+  CHP_HD f3& operator[](int k) const { return p[k * s]; }
 };
 
 // resolver kinds a context compiles: the batched narrowphase runs each pair class through a
@@ -80,9 +112,12 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   f4 ws_qa{};
   static constexpr int WS_NV = 32, WS_NE = 64;
   static constexpr int WS_VA = 0, WS_VB = WS_NV, WS_EA = 2 * WS_NV, WS_EB = WS_EA + WS_NE, WS_CA = WS_EB + WS_NE,
-                       WS_CB = WS_CA + CHP_MAXPOLY, WS_F3 = WS_CB + CHP_MAXPOLY;   // offsets in f3
-  CHP_HD NPCtx(const DevWorld<R>& w, R dt_, Sink<R, WAVE>* s, f3* ws_ = nullptr)
-      : W(w), dt(dt_), cm(0), sink(s), status(0), ws(ws_) {}
+                       WS_CB = WS_CA + CLIP_WAVE, WS_F3 = WS_CB + CLIP_WAVE;   // offsets in f3
+  // thread path: this lane's two clip polygons in an LDS area of 2 * CLIP_THREAD * 64 f3 shared
+  // by the wave (element k of polygon b at cb[(b * CLIP_THREAD + k) * 64 + lane])
+  f3* cb = nullptr;
+  CHP_HD NPCtx(const DevWorld<R>& w, R dt_, Sink<R, WAVE>* s, f3* ws_ = nullptr, f3* cb_ = nullptr)
+      : W(w), dt(dt_), cm(0), sink(s), status(0), ws(ws_), cb(cb_) {}
 
   // Narrowphase.createContactEquation (narrow_phase.dart:492-528) +
   // createFrictionEquationsFromContact (:530-586) + the World's body-material restitution
@@ -233,12 +268,16 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
       }
   }
 
-  // _pointInPolygon (:2584-2617) over world-space face vertices
-  CHP_HD static bool pointInPolygon(const f3* verts, int N, f3 normal, f3 p) {
+  // _pointInPolygon (:2584-2617) over the world-space vertices of hull face f0..f0+N (each
+  // computed when needed, with the same operations as a precomputed list)
+  CHP_HD static f3 face_vertex(const HullView<R>& H, int f0, int i, f3 xj, f4 qj) {
+    return add3<R>(xj, qrot<R>(qj, H.v[H.fidx[f0 + i]]));
+  }
+  CHP_HD static bool pointInPolygon(const HullView<R>& H, int f0, int N, f3 xj, f4 qj, f3 normal, f3 p) {
     int state = -1;
     for (int i = 0; i != N; i++) {
-      f3 v = verts[i];
-      f3 edge = sub3<R>(verts[(i + 1) % N], v);
+      f3 v = face_vertex(H, f0, i, xj, qj);
+      f3 edge = sub3<R>(face_vertex(H, f0, (i + 1) % N, xj, qj), v);
       f3 exn = cross3<R>(edge, normal);
       f3 vtp = sub3<R>(p, v);
       R r = dot3<R>(exn, vtp);
@@ -277,10 +316,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
       R pen = dot3<R>(pv, wn);
       f3 wp2s = sub3<R>(xi, wp);
       if (pen < (R)0 && dot3<R>(wp2s, wn) > (R)0) {
-        f3 fv[CHP_MAXPOLY];
-        if (nfv > CHP_MAXPOLY) { status |= WS_STAGING; return 0; }
-        for (int j = 0; j != nfv; j++) fv[j] = add3<R>(xj, qrot<R>(qj, H.v[H.fidx[f0 + j]]));
-        if (pointInPolygon(fv, nfv, wn, xi)) {
+        if (pointInPolygon(H, f0, nfv, xj, qj, wn, xi)) {
           f3 ri = scale3<R>(wn, -Rr);
           f3 n = neg3(wn);
           f3 pv2 = scale3<R>(wn, -pen);
@@ -320,6 +356,11 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   CHP_HD void planeConvex(const HullView<R>& H, f3 xi, f3 xj, f4 qi, f4 qj, int bi, int bj, SRef si, SRef sj,
                           SRef rsi, SRef rsj) {
     f3 wn = qrot<R>(qi, mk3(0.f, 0.f, 1.f));
+    if (sink->slot) {   // a direct sink: reserve the pair's contacts first
+      int k = 0;
+      for (int i = 0; i != H.nv; i++) k += dot3<R>(wn, sub3<R>(add3<R>(qrot<R>(qj, H.v[i]), xj), xi)) <= (R)0 ? 1 : 0;
+      sink->begin(k);
+    }
     for (int i = 0; i != H.nv; i++) {
       f3 wv = add3<R>(qrot<R>(qj, H.v[i]), xj);
       f3 rel = sub3<R>(wv, xi);
@@ -500,8 +541,8 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     if (dot3<R>(dc, target) > (R)0) target = neg3(target);
     return true;
   }
-  // clipFaceAgainstPlane (:545-587)
-  CHP_HD static int clipPlane(const f3* in, int nin, f3* out, f3 n, R c, int& status) {
+  // clipFaceAgainstPlane (:545-587); cap: the output polygon's capacity
+  CHP_HD static int clipPlane(Poly in, int nin, Poly out, int cap, f3 n, R c, int& status) {
     int nout = 0;
     if (nin < 2) return 0;
     f3 first = in[nin - 1];
@@ -510,18 +551,18 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
       f3 last = in[vi];
       R ndl = dot3<R>(n, last) + c;
       if (ndf < (R)0) {
-        if (ndl < (R)0) { if (nout < CHP_MAXPOLY) out[nout] = last; nout++; }
-        else { if (nout < CHP_MAXPOLY) out[nout] = lerp3<R>(first, last, ndf / (ndf - ndl)); nout++; }
+        if (ndl < (R)0) { if (nout < cap) out[nout] = last; nout++; }
+        else { if (nout < cap) out[nout] = lerp3<R>(first, last, ndf / (ndf - ndl)); nout++; }
       } else if (ndl < (R)0) {
-        if (nout < CHP_MAXPOLY) out[nout] = lerp3<R>(first, last, ndf / (ndf - ndl));
+        if (nout < cap) out[nout] = lerp3<R>(first, last, ndf / (ndf - ndl));
         nout++;
-        if (nout < CHP_MAXPOLY) out[nout] = last;
+        if (nout < cap) out[nout] = last;
         nout++;
       }
       first = last;
       ndf = ndl;
     }
-    if (nout > CHP_MAXPOLY) { status |= WS_STAGING; nout = CHP_MAXPOLY; }
+    if (nout > cap) { status |= WS_STAGING; nout = cap; }
     return nout;
   }
   // convexConvex (narrow_phase.dart:1981-2043) with clipAgainstHull / clipFaceAgainstHull
@@ -538,9 +579,19 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     if (dist3<R>(xi, xj) > A0.bsr + B0.bsr) return;
     HullView<R> A = A0, B = B0;
     f3 sep = mk3(0.f, 0.f, 0.f);
-    f3 lbuf[WAVE ? 1 : 2 * CHP_MAXPOLY];
-    f3* bufA = lbuf;
-    f3* bufB = lbuf + (WAVE ? 0 : CHP_MAXPOLY);
+    // the two clip polygons: the wave's workspace, or this lane's interleaved LDS area
+    Poly bufA, bufB;
+    int cap;
+    if constexpr (WAVE) {
+      bufA = Poly{ws + WS_CA, 1};
+      bufB = Poly{ws + WS_CB, 1};
+      cap = CLIP_WAVE;
+    } else {
+      const int lane = (int)(threadIdx.x & 63);
+      bufA = Poly{cb + lane, 64};
+      bufB = Poly{cb + CLIP_THREAD * 64 + lane, 64};
+      cap = CLIP_THREAD;
+    }
     if constexpr (WAVE) {
       // vertices and world edge directions from the workspace (the same values the global
       // tables give), clip polygons in it
@@ -564,8 +615,6 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
         weA = ws + WS_EA;
         weB = ws + WS_EB;
       }
-      bufA = ws + WS_CA;
-      bufB = ws + WS_CB;
       if (!findSepAxisWave(A, B, xi, qi, xj, qj, face0_only, sep, weA, weB)) return;
     } else {
       if (!findSepAxis(A, B, xi, qi, xj, qj, face0_only, sep)) return;
@@ -579,7 +628,9 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     }
     if (cfb < 0) return;
     int nin = B.fis[cfb + 1] - B.fis[cfb];
-    if (nin > CHP_MAXPOLY) { status |= WS_STAGING; return; }
+    if (nin > cap) { status |= WS_STAGING; return; }
+    // (wave path: every lane computes and stores the same polygon; a wave's LDS accesses
+    // complete in program order, so each lane reads back what the wave wrote)
     for (int k = 0; k < nin; k++) bufA[k] = add3<R>(qrot<R>(qj, B.v[B.fidx[B.fis[cfb] + k]]), xj);
     // clipFaceAgainstHull: A's face most anti-aligned with the axis
     int cfa = -1;
@@ -593,21 +644,29 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     // connected faces: faces sharing a vertex with the witness face, in face order (precomputed
     // per face with the hull)
     const int c0 = A.fcs[cfa], nconn = A.fcs[cfa + 1] - c0;
-    f3* pin = bufA;
-    f3* pout = bufB;
+    Poly pin = bufA, pout = bufB;
     for (int i = 0; i < nva; i++) {
       int other = (nconn > 0 && nconn > i) ? A.fconn[c0 + i] : 0;
       f3 lpn = A.fn[other];
       R lpe = -dot3<R>(lpn, A.v[A.fidx[A.fis[other]]]);
       f3 pnws = qrot<R>(qi, lpn);
       R pews = lpe - dot3<R>(pnws, xi);
-      nin = clipPlane(pin, nin, pout, pnws, pews, status);
-      f3* t = pin; pin = pout; pout = t;
+      nin = clipPlane(pin, nin, pout, cap, pnws, pews, status);
+      Poly t = pin; pin = pout; pout = t;
     }
     f3 lpn = A.fn[cfa];
     R lpe = -dot3<R>(lpn, A.v[A.fidx[pa0]]);
     f3 pnws = qrot<R>(qi, lpn);
     R pews = lpe - dot3<R>(pnws, xi);
+    if (sink->slot) {   // a direct sink: reserve the pair's contacts first
+      int k = 0;
+      for (int i = 0; i < nin; i++) {
+        R depth = dot3<R>(pnws, pin[i]) + pews;
+        if (depth <= (R)-100) depth = (R)-100;
+        k += (depth <= (R)100 && depth <= (R)1e-6) ? 1 : 0;
+      }
+      sink->begin(k);
+    }
     for (int i = 0; i < nin; i++) {
       R depth = dot3<R>(pnws, pin[i]) + pews;
       if (depth <= (R)-100) depth = (R)-100;

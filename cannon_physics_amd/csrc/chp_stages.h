@@ -514,8 +514,9 @@ template <typename R> __device__ void sap_merge(const DevWorld<R>& W, int g, int
 
 // ---- narrowphase ------------------------------------------------------------------------------
 // Pairs whose separating-axis search is large (more than WAVE_AXES axes: convex hulls without
-// unique axes, anything against heightfield pillars) run one pair per wave, the axes over the
-// lanes; every other pair runs one pair per thread.
+// unique axes, anything against heightfield pillars) or whose clip polygon may outgrow the
+// thread path's CLIP_THREAD points run one pair per wave, the axes over the lanes; every other
+// pair runs one pair per thread.
 constexpr int WAVE_AXES = 96;
 template <typename R> __device__ __forceinline__ bool wave_pair(const DevWorld<R>& W, int bi, int bj) {
   for (int a = 0; a < W.shape_count[bi]; a++)
@@ -528,7 +529,7 @@ template <typename R> __device__ __forceinline__ bool wave_pair(const DevWorld<R
         int ha = W.si_hull[sa], hb = W.si_hull[sb];
         int axes = (W.h_axes[ha] ? W.h_fcount[ha] : 0) + (W.h_axes[hb] ? W.h_fcount[hb] : 0) +
                    W.h_ecount[ha] * W.h_ecount[hb];
-        if (axes > WAVE_AXES) return true;
+        if (axes > WAVE_AXES || W.h_maxface[ha] + W.h_maxface[hb] > CLIP_THREAD) return true;
       }
     }
   return false;
@@ -548,16 +549,38 @@ template <typename R> __device__ __forceinline__ void pair_counts(const DevWorld
 }
 
 // one pair on one thread: contacts into the staging slab, per-pair counts.  KINDS: the
-// resolvers compiled in; KMAX: contacts buffered locally (a pair with more runs again straight
-// into the slab)
-template <typename R, unsigned KINDS = NK_ALL, int KMAX = CHP_KMAX>
-__device__ __forceinline__ void narrow_pair(const DevWorld<R>& W, int p, R dt) {
+// resolvers compiled in.  Buffered (DIRECT false): up to KMAX contacts are buffered locally and
+// then copied to a reserved range (a pair with more runs again straight into the slab).
+// DIRECT: the resolver (planeConvex, convexConvex: one call per pair) reserves its range
+// itself and writes there, with no local buffer.  cb: the wave's LDS clip-polygon area
+// (2 * CLIP_THREAD * 64 f3), needed when convexConvex is compiled in.
+template <typename R, unsigned KINDS = NK_ALL, int KMAX = CHP_KMAX, bool DIRECT = false>
+__device__ __forceinline__ void narrow_pair(const DevWorld<R>& W, int p, R dt, f3* cb) {
   int bi = W.pair_i[p], bj = W.pair_j[p];
   int n = 0, base = 0;
-  if (bi >= 0) {
+  if (bi < 0) {
+    pair_counts(W, p, 0, 0);
+    return;
+  }
+  if constexpr (DIRECT) {
+    Sink<R> s{W.contacts, W.contact_cap, 0, 0};
+    s.slot = &W.counters[C_STAGING];
+    NPCtx<R, false, KINDS> ctx(W, dt, &s, nullptr, cb);
+    ctx.pair(bi, bj);
+    if (ctx.status) ws_fail(W.counters, ctx.status);
+    if (s.overflow) {
+      ws_fail(W.counters, WS_CONTACTS);
+      s.n = s.nf = s.nc = 0;
+    }
+    W.pc_cnt[p] = s.n;
+    W.pc_base[p] = s.base;
+    W.pf_cnt[p] = s.nf;
+    W.pc_off[p] = s.nc;
+    return;
+  } else {
     ContactRec<R> loc[KMAX];
     Sink<R> s{loc, KMAX, 0, 0};
-    NPCtx<R, false, KINDS> ctx(W, dt, &s);
+    NPCtx<R, false, KINDS> ctx(W, dt, &s, nullptr, cb);
     ctx.pair(bi, bj);
     if (ctx.status) ws_fail(W.counters, ctx.status);
     n = s.n;
@@ -570,7 +593,7 @@ __device__ __forceinline__ void narrow_pair(const DevWorld<R>& W, int p, R dt) {
         for (int k = 0; k < n; k++) W.contacts[base + k] = loc[k];
       } else {  // more contacts than the local buffer: run the pair again into the slab
         Sink<R> s2{W.contacts + base, n, 0, 0};
-        NPCtx<R, false, KINDS> ctx2(W, dt, &s2);
+        NPCtx<R, false, KINDS> ctx2(W, dt, &s2, nullptr, cb);
         ctx2.pair(bi, bj);
       }
     }
@@ -579,9 +602,11 @@ __device__ __forceinline__ void narrow_pair(const DevWorld<R>& W, int p, R dt) {
 }
 
 // narrowphase of one world on one workgroup, one thread per pair (the slot entry point)
-template <typename R> __device__ void st_narrow(const DevWorld<R>& W, R dt) {
+// (cb: 2 * CLIP_THREAD * 64 f3 of LDS per wave of the workgroup)
+template <typename R> __device__ void st_narrow(const DevWorld<R>& W, R dt, f3* cb) {
   int P = W.counters[C_PAIRS];
-  for (int p = threadIdx.x; p < P; p += blockDim.x) narrow_pair(W, p, dt);
+  f3* wcb = cb + (threadIdx.x >> 6) * (2 * CLIP_THREAD * 64);
+  for (int p = threadIdx.x; p < P; p += blockDim.x) narrow_pair(W, p, dt, wcb);
   __syncthreads();
 }
 
@@ -707,21 +732,26 @@ constexpr unsigned cls_kinds(int c) {
          c == NP_SH ? NK_SH : c == NP_PC ? NK_PC : c == NP_CC ? NK_CC : NK_ALL;
 }
 constexpr int cls_kmax(int c) { return c <= NP_SC ? 1 : CHP_KMAX; }
+// classes whose single resolver call emits all of a pair's contacts at once write them straight
+// into the slab (no local buffer)
+constexpr bool cls_direct(int c) { return c == NP_PC || c == NP_CC; }
 
-// one thread class of the queue, one pair per thread (grid-stride)
+// one thread class of the queue, one pair per thread (grid-stride); cb: the wave's LDS clip area
 template <typename R, int CLS>
-__device__ void st_narrow_class(const DevWorld<R>* worlds, const int2* q, const int* bounds, R dt) {
+__device__ void st_narrow_class(const DevWorld<R>* worlds, const int2* q, const int* bounds, R dt, f3* cb) {
   const int b1 = bounds[CLS + 1];
   const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x), ng = (int)(gridDim.x * blockDim.x);
   for (int i = bounds[CLS] + g; i < b1; i += ng) {
     const int2 e = q[i];
-    narrow_pair<R, cls_kinds(CLS), cls_kmax(CLS)>(worlds[e.x], e.y, dt);
+    narrow_pair<R, cls_kinds(CLS), cls_kmax(CLS), cls_direct(CLS)>(worlds[e.x], e.y, dt, cb);
   }
 }
 
 // the queued separating-axis pairs, one wave per pair (grid-stride over the queue)
+// (ws: the wave's LDS workspace; loc: its LDS contact buffer of CHP_KMAX records)
 template <typename R, unsigned KINDS>
-__device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, R dt, f3* ws) {
+__device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, R dt, f3* ws,
+                               ContactRec<R>* loc) {
   const int lane = threadIdx.x & 63;
   const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
   for (int i = gw; i < cnt; i += nw) {
@@ -729,7 +759,6 @@ __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int
     const DevWorld<R>& W = worlds[q.x];
     const int p = q.y;
     const int bi = W.pair_i[p], bj = W.pair_j[p];
-    ContactRec<R> loc[CHP_KMAX];
     Sink<R, true> s{loc, CHP_KMAX, 0, 0};
     NPCtx<R, true, KINDS> ctx(W, dt, &s, ws);
     ctx.pair(bi, bj);
@@ -750,6 +779,10 @@ __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int
       for (int k = 0; k < n; k++) W.contacts[base + k] = loc[k];
     }
     if (lane == 0) pair_counts(W, p, n, base);
+    // the next pair reuses the workspace and the buffer
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 

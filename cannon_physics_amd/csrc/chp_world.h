@@ -33,6 +33,13 @@ enum : int {
 };
 
 enum : int { WS_OK = 0, WS_PAIRS = 1, WS_CONTACTS = 2, WS_ROWS = 4, WS_NONFINITE = 8, WS_STAGING = 16 };
+// WS_STAGING: a contact polygon outgrew its clipping buffer (faces are limited at world creation
+// so that this cannot happen; the bit is a guard)
+
+// clip polygon capacities: a face of nB vertices clipped by the nA side planes of the other
+// hull's face keeps at most nB + nA vertices.  Pairs whose largest faces sum to more than
+// CLIP_THREAD run on the wave path; hull faces are limited to CLIP_WAVE / 2 vertices.
+constexpr int CLIP_THREAD = 8, CLIP_WAVE = 64, MAX_FACE_VERTS = CLIP_WAVE / 2;
 
 // one solver row as the GS sweep reads it
 template <typename R> struct RowRec {
@@ -75,6 +82,7 @@ template <typename R> struct DevWorld {
   CHP_G(R) si_radius, si_bsr;
   // hull table
   CHP_G(int) h_vstart, h_vcount, h_fstart, h_fcount, h_estart, h_ecount, h_axes;
+  CHP_G(int) h_maxface;   // vertices of the hull's largest face (bounds its clip polygons)
   CHP_G(R) h_bsr;
   CHP_G(float) h_verts, h_fnormal, h_edges;
   CHP_G(int) f_istart, f_idx;

@@ -5,6 +5,8 @@
   c3_world          one world of the 4096-world batch: a box stack or a sphere/box pile
   c4_chain_world    one world of the 1024-world batch: a jointed chain of box links
   c5_terrain_world  one serving request: spheres/boxes/convex hulls over a value-noise terrain
+  feature_matrix_world  every body, shape, material and world option the recipe accepts, in
+                    one world (parity coverage, not a benchmark workload)
 
 Positions and sizes are drawn in double and rounded to FP32 on store, as the reference's
 Vector3 does.
@@ -14,9 +16,10 @@ import math
 import numpy as np
 
 from . import _abi as A
-from .objects import (Body, Box, ConeTwistConstraint, ConvexPolyhedron, DistanceConstraint,
-                      GSSolver, Heightfield, HingeConstraint, LockConstraint, NaiveBroadphase,
-                      Plane, PointToPointConstraint, Quaternion, SAPBroadphase, Sphere)
+from .objects import (Body, Box, ConeTwistConstraint, ContactMaterial, ConvexPolyhedron,
+                      DistanceConstraint, GSSolver, Heightfield, HingeConstraint, LockConstraint,
+                      Material, NaiveBroadphase, Plane, PointToPointConstraint, Quaternion,
+                      SAPBroadphase, Sphere)
 from .world import World
 
 
@@ -250,4 +253,121 @@ def c5_terrain_world(seed, n=None, grid=64, element_size=0.5, amplitude=2.0, **w
         else:
             w.addBody(Body(mass=1.0, position=(x, y, z), quaternion=shoemake_quaternion(rng),
                            shape=random_hull(rng, rng.uniform(0.2, 0.6))))
+    return w
+
+
+def feature_matrix_world(seed=7, broadphase="naive"):
+    """One world that sets every option of the scene recipe the benchmark workloads leave at
+    its default:
+      - body Materials (friction and restitution products; the World's restitution override of
+        contacts between two bodies with materials, world_class.dart:556-560), a Material with
+        only its restitution set, shape Materials and a ContactMaterial with its own stiffness
+        and relaxation (narrow_phase.dart:492-586, 660-695);
+      - collision filter groups / masks on bodies (broadphase.dart:44-63) and on shapes
+        (narrow_phase.dart:683-685);
+      - a trigger body, collisionResponse = false on a body and on a shape;
+      - a kinematic body moving through the pile (rigid_body.dart:303-314, 627-680);
+      - linearFactor / angularFactor, fixedRotation;
+      - World.frictionGravity and quatNormalizeSkip = 2."""
+    rng = np.random.default_rng(1_000_003 * 7 + seed)
+    bp = SAPBroadphase() if broadphase == "sap" else NaiveBroadphase()
+    w = World(gravity=(0, -9.82, 0), frictionGravity=(0, -6.0, 0), broadphase=bp,
+              solver=GSSolver(iterations=10), quatNormalizeSkip=2)
+    ground_mat = Material(friction=0.4, restitution=0.3)
+    ice = Material(friction=0.05, restitution=0.1)
+    rubber = Material(friction=0.9, restitution=0.8)
+    bouncy = Material(restitution=0.6)              # friction unset (-1)
+    w.addContactMaterial(ContactMaterial(ice, rubber, friction=0.2, restitution=0.5,
+                                         contactEquationStiffness=5e6, contactEquationRelaxation=4.0,
+                                         frictionEquationStiffness=2e6, frictionEquationRelaxation=5.0))
+    w.addContactMaterial(ContactMaterial(ground_mat, ice, friction=0.1, restitution=0.05))
+    g = ground_plane()
+    g.material = ground_mat
+    w.addBody(g)
+
+    def pos():
+        return (rng.uniform(-1.5, 1.5), rng.uniform(0.4, 3.0), rng.uniform(-1.5, 1.5))
+
+    # plain spheres and boxes with body materials
+    for k in range(10):
+        m = [rubber, ice, bouncy, None][k % 4]
+        if k % 2:
+            w.addBody(Body(mass=1.0, position=pos(), material=m, shape=Sphere(rng.uniform(0.2, 0.35))))
+        else:
+            w.addBody(Body(mass=1.5, position=pos(), material=m, quaternion=shoemake_quaternion(rng),
+                           shape=Box(rng.uniform(0.15, 0.3, size=3))))
+    # shape materials (the ContactMaterial lookup by shape materials) on boxes
+    for k in range(4):
+        b = Box(rng.uniform(0.15, 0.3, size=3))
+        b.material = ice if k % 2 else rubber
+        w.addBody(Body(mass=1.0, position=pos(), quaternion=shoemake_quaternion(rng), shape=b))
+    # filters: group 2 collides with group 1 only; group 4 with group 4 only (it falls through
+    # the plane's group 1 mask test)
+    for k in range(3):
+        w.addBody(Body(mass=1.0, position=pos(), collisionFilterGroup=2, collisionFilterMask=1,
+                       shape=Sphere(0.25)))
+    w.addBody(Body(mass=1.0, position=(0.3, 1.2, 0.2), collisionFilterGroup=4, collisionFilterMask=4,
+                   shape=Sphere(0.3)))
+    # shape-level filter: this sphere's shape only meets group-1 shapes... of group 8 shapes
+    fs = Sphere(0.3)
+    fs.collisionFilterGroup = 8
+    fs.collisionFilterMask = 8
+    w.addBody(Body(mass=1.0, position=pos(), shape=fs))
+    # trigger, no response on the body, no response on the shape
+    w.addBody(Body(mass=1.0, position=(-0.2, 0.5, 0.1), isTrigger=True, shape=Sphere(0.4)))
+    w.addBody(Body(mass=1.0, position=pos(), collisionResponse=False, shape=Box((0.25, 0.2, 0.3))))
+    ns = Sphere(0.3)
+    ns.collisionResponse = False
+    w.addBody(Body(mass=1.0, position=pos(), shape=ns))
+    # a kinematic box sweeping through the pile
+    w.addBody(Body(mass=2.0, type=2, position=(-2.0, 0.35, 0.0), velocity=(0.8, 0.0, 0.0),
+                   angularVelocity=(0.0, 0.7, 0.0), shape=Box((0.3, 0.3, 0.6))))
+    # factors and fixed rotation
+    w.addBody(Body(mass=1.0, position=pos(), linearFactor=(1, 0.5, 0), angularFactor=(0, 1, 0.5),
+                   quaternion=shoemake_quaternion(rng), shape=Box((0.2, 0.25, 0.3))))
+    w.addBody(Body(mass=1.0, position=pos(), fixedRotation=True, quaternion=shoemake_quaternion(rng),
+                   shape=Box((0.3, 0.2, 0.2))))
+    w.addBody(Body(mass=1.0, position=pos(), angularVelocity=(2.0, -1.0, 3.0), material=bouncy,
+                   shape=Box((0.2, 0.2, 0.2))))
+    if broadphase == "sap":
+        bp.autoDetectAxis()
+    return w
+
+
+def prism_hull(segments, radius, half_height):
+    """A right prism over a regular polygon of `segments` sides (a cylinder-like hull whose two
+    caps are single faces of `segments` vertices), faces wound counter-clockwise from outside."""
+    verts = []
+    for k in range(segments):
+        a = 2 * math.pi * k / segments
+        verts.append((radius * math.cos(a), -half_height, radius * math.sin(a)))
+    for k in range(segments):
+        a = 2 * math.pi * k / segments
+        verts.append((radius * math.cos(a), half_height, radius * math.sin(a)))
+    bottom = list(range(segments))                       # seen from below: counter-clockwise
+    top = [segments + k for k in reversed(range(segments))]
+    sides = [[k, segments + k, segments + (k + 1) % segments, (k + 1) % segments] for k in range(segments)]
+    return ConvexPolyhedron(vertices=verts, faces=[bottom, top] + sides)
+
+
+def prism_world(seed=9, segments=20):
+    """Prisms with 20-vertex caps stacked on boxes and on each other, boxes and spheres resting
+    on their caps: contact polygons larger than the thread path's clip buffers."""
+    rng = np.random.default_rng(1_000_003 * 9 + seed)
+    w = World(gravity=(0, -9.82, 0), broadphase=NaiveBroadphase(), solver=GSSolver(iterations=10))
+    w.addBody(ground_plane())
+    hull = prism_hull(segments, 0.5, 0.25)
+    for c in range(4):
+        x, z = 1.5 * c, rng.uniform(-0.05, 0.05)
+        y = 0.25 + 0.002
+        w.addBody(Body(mass=1.0, position=(x, y, z), shape=hull))
+        y += 0.25 + 0.3 + 0.003
+        w.addBody(Body(mass=1.0, position=(x + rng.uniform(-0.1, 0.1), y, z), shape=Box((0.35, 0.3, 0.35))))
+        y += 0.3 + 0.25 + 0.003
+        w.addBody(Body(mass=1.0, position=(x, y, z), quaternion=Quaternion().setFromEuler(0, rng.uniform(0, 1), 0),
+                       shape=hull))
+        y += 0.25 + 0.25 + 0.003
+        w.addBody(Body(mass=1.0, position=(x + 0.1, y, z), shape=hull))
+        y += 0.25 + 0.2 + 0.01
+        w.addBody(Body(mass=0.5, position=(x - 0.1, y, z + 0.1), shape=Sphere(0.2)))
     return w

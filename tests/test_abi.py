@@ -63,7 +63,8 @@ def _inspect(world, precision=A.PRECISION_F32):
 
 def test_inspect_accepts_every_scene_family():
     for w in (scenes.c1_spheres(), scenes.c3_world(3), scenes.c4_chain_world(5),
-              scenes.c2_large_world(n=64, extent=(8.0, 4.0, 8.0)), scenes.c5_terrain_world(1, n=8)):
+              scenes.c2_large_world(n=64, extent=(8.0, 4.0, 8.0)), scenes.c5_terrain_world(1, n=8),
+              scenes.feature_matrix_world(7), scenes.prism_world(9)):
         rc, msg = _inspect(w)
         assert rc == A.CHP_OK, msg
 
@@ -98,3 +99,20 @@ def test_context_without_device_fails_cleanly():
         pytest.skip("a gfx950 device is present")
     assert rc == A.CHP_E_NODEVICE
     assert not p.value
+
+
+def test_hull_faces_beyond_the_clip_limit_are_rejected_clearly():
+    """A contact polygon (a face of n vertices clipped by the m side planes of another face)
+    has at most n + m points; the device clips in buffers of 64 points, so faces are limited to
+    32 vertices and a larger one is refused at construction with a message that says so."""
+    from cannon_physics_amd.objects import Body
+    from cannon_physics_amd.world import World
+    for segs, ok in ((32, True), (33, False)):
+        w = World(gravity=(0, -9.82, 0))
+        w.addBody(scenes.ground_plane())
+        w.addBody(Body(mass=1, position=(0, 1, 0), shape=scenes.prism_hull(segs, 0.5, 0.2)))
+        rc, msg = _inspect(w)
+        if ok:
+            assert rc == A.CHP_OK, msg
+        else:
+            assert rc == A.CHP_E_INVALID and "33 vertices" in msg and "32" in msg, msg

@@ -28,6 +28,13 @@ SCENES = {
     "c4_mixed": (lambda: scenes.c4_chain_world(15, length=30), 1 / 120, 40),
     "c2_sap": (lambda: scenes.c2_large_world(n=800, extent=(14.0, 8.0, 14.0)), DT, 40),
     "c5_terrain": (lambda: scenes.c5_terrain_world(4, n=40), DT, 50),
+    # materials, contact materials, restitution override, filters, trigger, disabled response,
+    # kinematic body, linear / angular factors, fixed rotation, frictionGravity,
+    # quatNormalizeSkip = 2
+    "feature_matrix": (lambda: scenes.feature_matrix_world(7), DT, 45),
+    "feature_matrix_sap": (lambda: scenes.feature_matrix_world(8, "sap"), DT, 45),
+    # 20-vertex prism caps: contact polygons beyond the thread path's clip buffers (wave path)
+    "prism20": (lambda: scenes.prism_world(9), DT, 40),
 }
 
 _ctx = {}

@@ -20,6 +20,7 @@ SCENES = {
     "c3_pile": lambda: scenes.c3_world(12, kind="pile"),
     "c4_chain": lambda: scenes.c4_chain_world(7, length=25),
     "c5_terrain": lambda: scenes.c5_terrain_world(3, n=24),
+    "feature_matrix": lambda: scenes.feature_matrix_world(7),
 }
 
 
