@@ -44,6 +44,7 @@ def load():
         "chp_state_upload": (i32, [ctx_p, u32, P(A.BodyState), u32]),
         "chp_step": (i32, [ctx_p, P(u32), u32, dbl, i32]),
         "chp_step_plain": (i32, [ctx_p, u32, dbl, i32]),
+        "chp_step_world_noalloc": (i32, [ctx_p, u32, dbl, i32]),
         "chp_step_async": (i32, [ctx_p, P(u32), u32, dbl, i32]),
         "chp_sync": (i32, [ctx_p]),
         "chp_last_step_ms": (i32, [ctx_p, P(dbl)]),

@@ -557,6 +557,8 @@ struct chp_ctx {
   int* d_list = nullptr;
   size_t d_list_cap = 0;
   std::vector<int> h_list;  // host copy of d_list (the source of its last upload)
+  void* scratch = nullptr;  // this device's no-allocation scratch array
+  std::vector<int> na_host; // host source of the no-allocation path's table uploads
   bool list_valid = false;  // d_list and the large-world tables hold h_list's tables
   std::vector<int> prev_list;   // the list and dt of the previous chp_step call
   double prev_dt = 0;
@@ -813,6 +815,22 @@ static bool valid_world(chp_ctx* ctx, uint32_t id) { return ctx && id < ctx->wor
 
 template <typename R> static DevWorld<R>& dev_of(WorldSlot& S) { return *(DevWorld<R>*)S.devworld.data(); }
 
+// the damping factors of a world for dt (world_class.dart:652-656: pow(1 - damping, dt),
+// evaluated once per (body, dt) on the host); no device allocation
+template <typename R> static int set_damping(chp_ctx* ctx, WorldSlot& S, double dt) {
+  if (S.last_dt == dt) return CHP_OK;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<R> ldv(S.nb), adv(S.nb);
+  for (int i = 0; i < S.nb; i++) {
+    ldv[i] = (R)std::pow((double)((R)1 - (R)S.lin_damp[i]), (double)(R)dt);
+    adv[i] = (R)std::pow((double)((R)1 - (R)S.ang_damp[i]), (double)(R)dt);
+  }
+  HIPCHK(ctx, hipMemcpy((char*)S.mem + S.off_ldamp, ldv.data(), S.nb * sizeof(R), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy((char*)S.mem + S.off_adamp, adv.data(), S.nb * sizeof(R), hipMemcpyHostToDevice));
+  S.last_dt = dt;
+  return CHP_OK;
+}
+
 // uploads the DevWorld table and the damping factors for dt
 template <typename R> static int prepare(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt) {
   if (ctx->worlds_dirty || ctx->d_worlds_cap < ctx->worlds.size()) {
@@ -833,18 +851,8 @@ template <typename R> static int prepare(chp_ctx* ctx, const uint32_t* ids, uint
     ctx->worlds_dirty = false;
   }
   for (uint32_t k = 0; k < n; k++) {
-    WorldSlot& S = ctx->worlds[ids[k]];
-    if (S.last_dt == dt) continue;
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    // world_class.dart:652-656: pow(1 - damping, dt), evaluated once per (body, dt)
-    std::vector<R> ldv(S.nb), adv(S.nb);
-    for (int i = 0; i < S.nb; i++) {
-      ldv[i] = (R)std::pow((double)((R)1 - (R)S.lin_damp[i]), (double)(R)dt);
-      adv[i] = (R)std::pow((double)((R)1 - (R)S.ang_damp[i]), (double)(R)dt);
-    }
-    HIPCHK(ctx, hipMemcpy((char*)S.mem + S.off_ldamp, ldv.data(), S.nb * sizeof(R), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy((char*)S.mem + S.off_adamp, adv.data(), S.nb * sizeof(R), hipMemcpyHostToDevice));
-    S.last_dt = dt;
+    int r = set_damping<R>(ctx, ctx->worlds[ids[k]], dt);
+    if (r != CHP_OK) return r;
   }
   return CHP_OK;
 }
@@ -1268,6 +1276,94 @@ template <typename R> static int step_plain_t(chp_ctx* ctx, uint32_t id, double 
   return check_status(ctx, &id, 1);
 }
 
+// ---- the allocation-free step ------------------------------------------------------------------
+// chp_step_world_noalloc carves every buffer a step launches over (the DevWorld table entry,
+// the world list, the large-world tables, the narrowphase class counts and queue) from this
+// fixed array instead of allocating them.  Each device the library runs on holds its own copy
+// until the process ends; every call on a device shares it (calls that overlap on one device
+// race).  Its device address is looked up once per device, with that device current.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::map<int, void*> g_scratch_ptr;   // device ordinal -> address of that device's array
+
+// the address of the device's array (the device must be current); nullptr on failure
+static void* scratch_address(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+  g_scratch_ptr[device] = p;
+  return p;
+}
+
+// one world, nsteps steps, every buffer from the scratch array (no device allocation); one
+// host synchronisation at the end.  Everything the kernels read from the array is written
+// before it is read in the same call (the world's table entry, its list and large-world
+// tables are uploaded, the class counts and the queue are written by the narrowphase passes).
+template <typename R> static int step_noalloc_t(chp_ctx* ctx, uint32_t id, double dt, int nsteps) {
+  if (!ctx->scratch) return fail(ctx, CHP_E_HIP, "the no-allocation scratch array has no device address");
+  WorldSlot& S = ctx->worlds[id];
+  const bool big = S.nb >= BP_BIG;
+  const int nspans = big ? (S.nb + BP_SPAN - 1) / BP_SPAN : 0;
+  const int ntiles = (big && S.broadphase == CHP_BROADPHASE_SAP) ? (S.nb + SAP_TILE - 1) / SAP_TILE : 0;
+  int rounds = 0;
+  while ((1 << rounds) < std::max(1, ntiles)) rounds++;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = (off + 255) & ~(size_t)255;
+    off = o + std::max<size_t>(bytes, 1);
+    return o;
+  };
+  const size_t nbig_ints = 2 * (size_t)nspans + 2 * (size_t)ntiles + (big ? 1 : 0);
+  const size_t o_tab = take(sizeof(DevWorld<R>)), o_list = take(sizeof(int)), o_big = take(nbig_ints * sizeof(int)),
+               o_qcls = take((2 * NP_NCLS + 1) * sizeof(int)), o_q = take((size_t)S.pair_cap * sizeof(int2));
+  if (off > NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, CHP_E_CAPACITY, "world " + std::to_string(id) + " needs " + std::to_string(off) +
+                                         " bytes of step buffers; the no-allocation path holds " +
+                                         std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                         " (lower its pair capacity or use chp_step)");
+  int r = set_damping<R>(ctx, S, dt);
+  if (r != CHP_OK) return r;
+  // host images of the small tables: the stream is idle before they change
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int>& h = ctx->na_host;
+  h.assign(1 + nbig_ints, 0);
+  for (int k = 0; k < nspans; k++) { h[1 + 2 * k] = 0; h[2 + 2 * k] = k * BP_SPAN; }
+  for (int k = 0; k < ntiles; k++) { h[1 + 2 * nspans + 2 * k] = 0; h[2 + 2 * nspans + 2 * k] = k * SAP_TILE; }
+  char* base = (char*)ctx->scratch;
+  HIPCHK(ctx, hipMemcpyAsync(base + o_tab, S.devworld.data(), sizeof(DevWorld<R>), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(base + o_list, h.data(), sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  if (nbig_ints)
+    HIPCHK(ctx, hipMemcpyAsync(base + o_big, h.data() + 1, nbig_ints * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  StepView V;
+  V.worlds = base + o_tab;
+  V.list = (const int*)(base + o_list);
+  V.big = (const int*)(base + o_big);
+  V.nspans = nspans; V.ntiles = ntiles; V.nw = big ? 1 : 0; V.rounds = rounds;
+  V.n_sap = S.broadphase == CHP_BROADPHASE_SAP ? 1 : 0;
+  V.qcls = (int*)(base + o_qcls);
+  V.q = (int2*)(base + o_q);
+  V.qcap = (size_t)S.pair_cap;
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int s = 0; s < nsteps; s++) {
+    r = launch_step<R>(ctx, V, 1, S.nb > SMALL_NB ? 1 : 0, S.nb, dt, nullptr);
+    if (r != CHP_OK) return r;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  ctx->timed = true;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  // the world's status word, read straight from its slab
+  int st[C_COUNT];
+  HIPCHK(ctx, hipMemcpy(st, (char*)S.mem + S.off_counters, sizeof st, hipMemcpyDeviceToHost));
+  if (st[C_STATUS] & WS_NONFINITE) return fail(ctx, CHP_E_NONFINITE, "world " + std::to_string(id) + " produced non-finite state");
+  if (st[C_STATUS])
+    return fail(ctx, CHP_E_CAPACITY, "world " + std::to_string(id) + " exceeded a slab (status " +
+                                         std::to_string(st[C_STATUS]) + "); raise it with chp_world_set_capacity");
+  return CHP_OK;
+}
+
 template <typename R> static int set_capacity_t(chp_ctx* ctx, uint32_t id, uint32_t mp, uint32_t mc, uint32_t mr) {
   WorldSlot& S = ctx->worlds[id];
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1676,6 +1772,7 @@ int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out
   c->max_lds = (size_t)prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CHP_E_HIP; }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return CHP_E_HIP; }
+  c->scratch = scratch_address(device);   // the device is current; checked on use
 
   *out = c;
   return CHP_OK;
@@ -1811,6 +1908,17 @@ int chp_step_plain(chp_ctx* ctx, uint32_t id, double dt, int32_t nsteps) {
   (void)hipSetDevice(ctx->device);
   return ctx->precision == CHP_PRECISION_F64 ? step_plain_t<double>(ctx, id, dt, nsteps)
                                              : step_plain_t<float>(ctx, id, dt, nsteps);
+}
+
+int chp_step_world_noalloc(chp_ctx* ctx, uint32_t id, double dt, int32_t nsteps) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!(dt > 0) || nsteps < 0) return fail(ctx, CHP_E_INVALID, "dt must be > 0 and nsteps >= 0");
+  if (nsteps == 0) return CHP_OK;
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? step_noalloc_t<double>(ctx, id, dt, nsteps)
+                                             : step_noalloc_t<float>(ctx, id, dt, nsteps);
 }
 
 int chp_world_stats(chp_ctx* ctx, uint32_t id, chp_step_stats* out) {

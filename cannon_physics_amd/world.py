@@ -117,6 +117,10 @@ class Context:
     def step_plain(self, wid, dt, nsteps=1):
         self.check(self.lib.chp_step_plain(self.ptr, wid, dt, nsteps))
 
+    def step_noalloc(self, wid, dt, nsteps=1):
+        """chp_step_world_noalloc: one world's steps with no device allocation"""
+        self.check(self.lib.chp_step_world_noalloc(self.ptr, wid, dt, nsteps))
+
     def stats(self, wid):
         s = A.StepStats()
         self.check(self.lib.chp_world_stats(self.ptr, wid, C.byref(s)))

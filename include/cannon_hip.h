@@ -248,6 +248,14 @@ int chp_step(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, double dt, 
 /* Plain path: one world, one launch per stage per step and a host synchronisation per step.
  * Same results as chp_step. */
 int chp_step_plain(chp_ctx* ctx, uint32_t world_id, double dt, int32_t nsteps);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc): the same steps as chp_step_plain for one
+ * world, launched back to back with one synchronisation at the end, with no device allocation
+ * and no allocating library call.  Every buffer the step launches over comes from a fixed
+ * scratch array of 64 MiB that each device holds until the process ends: a world whose step
+ * buffers (chiefly 8 bytes per pair of its pair capacity) exceed it is refused with
+ * CHP_E_CAPACITY.  Calls on one device share the array; calls that overlap on one device race. */
+int chp_step_world_noalloc(chp_ctx* ctx, uint32_t world_id, double dt, int32_t nsteps);
 /* Asynchronous batched step on the context's stream; chp_sync waits for it. */
 int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, double dt,
                    int32_t nsteps);
