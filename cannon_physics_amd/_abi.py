@@ -22,6 +22,7 @@ BROADPHASE_NAIVE, BROADPHASE_SAP = 0, 1
  CONSTRAINT_CONE_TWIST) = range(5)
 ROW_FRICTION, ROW_CONTACT, ROW_ROTATIONAL, ROW_CONE, ROW_MOTOR = range(5)
 PRECISION_F32, PRECISION_F64 = 0, 1
+SLEEP_AWAKE, SLEEP_SLEEPY, SLEEP_SLEEPING = 0, 1, 2
 
 F3 = C.c_float * 3
 F4 = C.c_float * 4
@@ -46,6 +47,8 @@ class WorldDesc(C.Structure):
         ("contact_relaxation", C.c_double),
         ("friction_stiffness", C.c_double),
         ("friction_relaxation", C.c_double),
+        ("allow_sleep", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -131,6 +134,10 @@ class BodyDesc(C.Structure):
         ("quaternion_after", F4),
         ("set_position", C.c_int32),
         ("position_after", F3),
+        ("allow_sleep", C.c_int32),
+        ("sleep_state", C.c_int32),
+        ("sleep_speed_limit", C.c_double),
+        ("sleep_time_limit", C.c_double),
     ]
 
 
@@ -187,7 +194,8 @@ class BodyState(C.Structure):
         ("aabb_lower", F3),
         ("aabb_upper", F3),
         ("aabb_needs_update", C.c_int32),
-        ("reserved", C.c_int32),
+        ("sleep_state", C.c_int32),
+        ("time_last_sleepy", C.c_double),
     ]
 
 
@@ -272,6 +280,9 @@ class StepStats(C.Structure):
         ("sum_pairs", C.c_uint32),
         ("sum_rows", C.c_uint32),
         ("sum_contacts", C.c_uint32),
+        ("sleeping", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("time", C.c_double),
     ]
 
 

@@ -55,6 +55,7 @@ def load():
         "chp_ctx_set_profiling": (i32, [ctx_p, C.c_int]),
         "chp_stage_times": (i32, [ctx_p, P(dbl), P(C.c_char_p), u32, P(u32)]),
         "chp_world_stats": (i32, [ctx_p, u32, P(A.StepStats)]),
+        "chp_world_set_time": (i32, [ctx_p, u32, dbl]),
         "chp_broadphase": (i32, [ctx_p, u32, P(i32), P(i32), u32, P(u32)]),
         "chp_step_capture": (i32, [ctx_p, u32, dbl, P(i32), P(i32), u32, P(u32), P(A.Row), u32,
                                    P(u32), P(i32)]),

@@ -377,6 +377,14 @@ __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worl
   }
 }
 
+// the wake-up pass after the narrowphase (worlds that may hold sleeping bodies)
+template <typename R>
+__global__ __launch_bounds__(256) void k_wake(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list) {
+  __shared__ int sh[256];
+  CHP_WORLD_PROLOGUE
+  st_wake(W);
+}
+
 // velocities, damping, integration, clearForces
 template <typename R>
 __global__ __launch_bounds__(256) void k_post(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
@@ -389,11 +397,11 @@ __global__ __launch_bounds__(256) void k_post(const DevWorld<R>* __restrict__ wo
         W.counters[C_POSTFAIL] = 0;
         ws_fail(W.counters, WS_NONFINITE);
       }
-      W.counters[C_STEP] += 1;
+      step_done(W, dt);
     }
     return;
   }
-  st_post(W, (R)dt);
+  st_post(W, dt);
 }
 
 // broadphase only (Broadphase.collisionPairs on the current state)
@@ -532,6 +540,7 @@ struct WorldSlot {
   void* mem = nullptr;
   size_t bytes = 0;
   double last_dt = -1;
+  bool may_sleep = false;  // World.allowSleep, or a body asleep: the step runs the wake-up pass
   std::vector<double> lin_damp, ang_damp;  // Body.linearDamping / angularDamping
   std::vector<chp_body_info> info;
   int pair_cap = 0, contact_cap = 0, row_cap = 0;
@@ -568,6 +577,7 @@ struct chp_ctx {
   std::vector<int> h_big;
   int big_nspans = 0, big_ntiles = 0, big_nw = 0, big_rounds = 0;
   int n_sap = 0;   // SAP worlds in the current list
+  int n_sleep = 0; // worlds in the current list that may hold sleeping bodies
   bool keep_rows = false;   // steps also write row records (the capture reads them)
   int* d_status = nullptr;
   const int** d_cptrs = nullptr;
@@ -590,7 +600,7 @@ struct chp_ctx {
     bool keep = false;
     double dt = 0;
     const void *dq = nullptr, *dqcls = nullptr, *dlist = nullptr, *dworlds = nullptr, *dbig = nullptr;
-    std::array<int, 5> big{};
+    std::array<int, 6> big{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } sg;
@@ -662,6 +672,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   D.quat_skip = H.wd.quat_normalize_skip;
   D.quat_fast = H.wd.quat_normalize_fast;
   D.has_filtered_constraints = H.has_filtered ? 1 : 0;
+  D.allow_sleep = H.wd.allow_sleep ? 1 : 0;
   for (int k = 0; k < 3; k++) D.gravity[k] = H.wd.gravity[k];
   {
     f3 g = H.wd.has_friction_gravity ? mk3(H.wd.friction_gravity[0], H.wd.friction_gravity[1], H.wd.friction_gravity[2])
@@ -689,6 +700,12 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   fix.emplace_back((void**)(void*)&D.adamp, o_ad);
   ARR(type, H.type); ARR(flags, H.flags); ARR(group, H.group); ARR(mask, H.mask); ARR(material, H.material);
   ARR(shape_first, H.shape_first); ARR(shape_count, H.shape_count); ARR(order, H.order);
+  ARR(sleep_state, H.sleep_state); ARR(sleep_tls, H.sleep_tls); ARR(sleep_speed, H.sleep_speed);
+  ARR(sleep_time, H.sleep_time);
+  {
+    std::vector<double> t0(1, H.time);
+    ARR(wtime, t0);
+  }
   ARR(si_type, H.si_type); ARR(si_hull, H.si_hull); ARR(si_hf, H.si_hf); ARR(si_response, H.si_response);
   ARR(si_group, H.si_group); ARR(si_mask, H.si_mask); ARR(si_material, H.si_material);
   ARR(si_off, H.si_off); ARR(si_ori, H.si_ori); ARR(si_half, H.si_half); ARR(si_radius, H.si_radius); ARR(si_bsr, H.si_bsr);
@@ -745,6 +762,8 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   S.pair_cap = pair_cap; S.contact_cap = contact_cap; S.row_cap = row_cap;
   S.off_ldamp = o_ld; S.off_adamp = o_ad; S.off_counters = o_cnt;
   S.last_dt = -1;
+  S.may_sleep = H.wd.allow_sleep != 0;
+  for (int v : H.sleep_state) S.may_sleep = S.may_sleep || v == CHP_SLEEP_SLEEPING;
   S.devworld.assign((char*)&D, (char*)&D + sizeof(D));
   ctx->worlds_dirty = true;
   ctx->list_valid = false;
@@ -948,9 +967,11 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   std::vector<int> spans, tiles, bigw;
   int max_tiles = 1;
   ctx->n_sap = 0;
+  ctx->n_sleep = 0;
   for (size_t li = 0; li < n; li++) {
     const WorldSlot& S = ctx->worlds[list[li]];
     if (S.broadphase == CHP_BROADPHASE_SAP) ctx->n_sap++;
+    if (S.may_sleep) ctx->n_sleep++;
     if (S.nb < BP_BIG) continue;
     bigw.push_back((int)li);
     for (int s0 = 0; s0 < S.nb; s0 += BP_SPAN) { spans.push_back((int)li); spans.push_back(s0); }
@@ -992,7 +1013,7 @@ struct StepView {
   const void* worlds = nullptr;
   const int* list = nullptr;
   const int* big = nullptr;
-  int nspans = 0, ntiles = 0, nw = 0, rounds = 0, n_sap = 0;
+  int nspans = 0, ntiles = 0, nw = 0, rounds = 0, n_sap = 0, n_sleep = 0;
   int2* q = nullptr;
   size_t qcap = 0;
   int* qcls = nullptr;
@@ -1007,6 +1028,7 @@ static StepView ctx_view(const chp_ctx* ctx) {
   v.big = ctx->d_big;
   v.nspans = ctx->big_nspans; v.ntiles = ctx->big_ntiles; v.nw = ctx->big_nw; v.rounds = ctx->big_rounds;
   v.n_sap = ctx->n_sap;
+  v.n_sleep = ctx->n_sleep;
   v.q = ctx->d_q;
   v.qcap = ctx->d_q_cap;
   v.qcls = ctx->d_qcls;
@@ -1078,6 +1100,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
     k_narrow_wave<R, NP_WHC><<<(waves + 3) / 4, 256, 0, st>>>(dw, V.q, bounds, dt);
     // compound-body pairs are rare: a small grid, strided over however many there are
     k_narrow_wave<R, NP_WAVE><<<32, 256, 0, st>>>(dw, V.q, bounds, dt);
+    if (V.n_sleep) k_wake<R><<<n, 256, 0, st>>>(dw, d_list);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
   // rows: offsets, then the rows' body pairs; the level schedule; then the rows, written as GS
@@ -1142,7 +1165,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
   *out = nullptr;
   if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block && g.keep == ctx->keep_rows &&
         g.bg == ctx->solve_block && g.dq == ctx->d_q && g.dqcls == ctx->d_qcls && g.dlist == ctx->d_list &&
-        g.dbig == ctx->d_big && g.big == std::array<int, 5>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap} &&
+        g.dbig == ctx->d_big && g.big == std::array<int, 6>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep} &&
         g.dworlds == ctx->d_worlds)) {
     if (!may_capture) return CHP_OK;
     drop_graph(ctx);
@@ -1161,7 +1184,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
     g.valid = true; g.list = list; g.nbig = nbig; g.max_nb = max_nb; g.dt = dt;
     g.bl = ctx->levels_block; g.bg = ctx->solve_block; g.keep = ctx->keep_rows;
     g.dq = ctx->d_q; g.dqcls = ctx->d_qcls; g.dlist = ctx->d_list; g.dworlds = ctx->d_worlds;
-    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap};
+    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep};
     g.graph = graph;
     g.exec = exec;
   }
@@ -1343,6 +1366,7 @@ template <typename R> static int step_noalloc_t(chp_ctx* ctx, uint32_t id, doubl
   V.big = (const int*)(base + o_big);
   V.nspans = nspans; V.ntiles = ntiles; V.nw = big ? 1 : 0; V.rounds = rounds;
   V.n_sap = S.broadphase == CHP_BROADPHASE_SAP ? 1 : 0;
+  V.n_sleep = S.may_sleep ? 1 : 0;
   V.qcls = (int*)(base + o_qcls);
   V.q = (int2*)(base + o_q);
   V.qcap = (size_t)S.pair_cap;
@@ -1377,7 +1401,9 @@ template <typename R> static int set_capacity_t(chp_ctx* ctx, uint32_t id, uint3
   hipError_t e = hipSuccess;
 #define G(field) if (e == hipSuccess) e = get(H.field, D.field)
   G(pos); G(quat); G(vel); G(avel); G(force); G(torque); G(iiw); G(aabb_lo); G(aabb_hi); G(flags); G(order);
+  G(sleep_state); G(sleep_tls);
 #undef G
+  if (e == hipSuccess) e = hipMemcpy(&H.time, D.wtime, sizeof(double), hipMemcpyDeviceToHost);
   int counters[C_COUNT];
   if (e == hipSuccess) e = hipMemcpy(counters, D.counters, sizeof counters, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return fail(ctx, CHP_E_HIP, std::string("capacity change: ") + hipGetErrorString(e));
@@ -1404,13 +1430,16 @@ template <typename R> static int state_io(chp_ctx* ctx, uint32_t id, chp_body_st
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<float> pos(3 * nb), quat(4 * nb), vel(3 * nb), avel(3 * nb), force(3 * nb), torque(3 * nb), iiw(9 * nb),
       lo(3 * nb), hi(3 * nb);
-  std::vector<int> flags(nb);
+  std::vector<int> flags(nb), sleep(nb);
+  std::vector<double> tls(nb);
   struct F { std::vector<float>* v; float* d; } fs[] = {{&pos, D.pos}, {&quat, D.quat}, {&vel, D.vel}, {&avel, D.avel},
                                                        {&force, D.force}, {&torque, D.torque}, {&iiw, D.iiw},
                                                        {&lo, D.aabb_lo}, {&hi, D.aabb_hi}};
   if (download) {
     for (auto& f : fs) HIPCHK(ctx, hipMemcpy(f.v->data(), f.d, f.v->size() * 4, hipMemcpyDeviceToHost));
     HIPCHK(ctx, hipMemcpy(flags.data(), D.flags, nb * 4, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(sleep.data(), D.sleep_state, nb * 4, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(tls.data(), D.sleep_tls, nb * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < nb; i++) {
       chp_body_state& s = buf[i];
       memset(&s, 0, sizeof(s));
@@ -1420,11 +1449,20 @@ template <typename R> static int state_io(chp_ctx* ctx, uint32_t id, chp_body_st
       memcpy(s.inv_inertia_world, &iiw[9 * i], 36);
       memcpy(s.aabb_lower, &lo[3 * i], 12); memcpy(s.aabb_upper, &hi[3 * i], 12);
       s.aabb_needs_update = (flags[i] & BF_AABB_DIRTY) ? 1 : 0;
+      s.sleep_state = sleep[i];
+      s.time_last_sleepy = tls[i];
     }
   } else {
+    for (uint32_t i = 0; i < (uint32_t)nb; i++)
+      if (buf[i].sleep_state < CHP_SLEEP_AWAKE || buf[i].sleep_state > CHP_SLEEP_SLEEPING)
+        return fail(ctx, CHP_E_INVALID, "body " + std::to_string(i) + ": sleep_state is not a CHP_SLEEP_* value");
     HIPCHK(ctx, hipMemcpy(flags.data(), D.flags, nb * 4, hipMemcpyDeviceToHost));
+    bool asleep = false;
     for (int i = 0; i < nb; i++) {
       const chp_body_state& s = buf[i];
+      sleep[i] = s.sleep_state;
+      tls[i] = s.time_last_sleepy;
+      asleep = asleep || s.sleep_state == CHP_SLEEP_SLEEPING;
       memcpy(&pos[3 * i], s.position, 12); memcpy(&quat[4 * i], s.quaternion, 16);
       memcpy(&vel[3 * i], s.velocity, 12); memcpy(&avel[3 * i], s.angular_velocity, 12);
       memcpy(&force[3 * i], s.force, 12); memcpy(&torque[3 * i], s.torque, 12);
@@ -1434,9 +1472,34 @@ template <typename R> static int state_io(chp_ctx* ctx, uint32_t id, chp_body_st
     }
     for (auto& f : fs) HIPCHK(ctx, hipMemcpy(f.d, f.v->data(), f.v->size() * 4, hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(D.flags, flags.data(), nb * 4, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(D.sleep_state, sleep.data(), nb * 4, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(D.sleep_tls, tls.data(), nb * 8, hipMemcpyHostToDevice));
+    const bool may = D.allow_sleep != 0 || asleep;
+    if (may != S.may_sleep) {   // the step's wake-up pass is launched per list
+      S.may_sleep = may;
+      ctx->list_valid = false;
+    }
     int zero = 0;  // a fresh state clears a quarantine
     HIPCHK(ctx, hipMemcpy(D.counters + C_STATUS, &zero, 4, hipMemcpyHostToDevice));
   }
+  return CHP_OK;
+}
+
+// the sleeping-body count and World.time of a world
+template <typename R> static int sleep_stats_t(chp_ctx* ctx, uint32_t id, chp_step_stats* out) {
+  WorldSlot& S = ctx->worlds[id];
+  DevWorld<R>& D = dev_of<R>(S);
+  std::vector<int> sleep(S.nb);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (S.nb) HIPCHK(ctx, hipMemcpy(sleep.data(), D.sleep_state, S.nb * 4, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(&out->time, D.wtime, sizeof(double), hipMemcpyDeviceToHost));
+  out->sleeping = (uint32_t)std::count(sleep.begin(), sleep.end(), (int)CHP_SLEEP_SLEEPING);
+  return CHP_OK;
+}
+template <typename R> static int set_time_t(chp_ctx* ctx, uint32_t id, double time) {
+  DevWorld<R>& D = dev_of<R>(ctx->worlds[id]);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(D.wtime, &time, sizeof(double), hipMemcpyHostToDevice));
   return CHP_OK;
 }
 
@@ -1746,6 +1809,8 @@ template <typename R> static int inspect_t(const chp_scene* scene, chp_body_info
       memcpy(s.inv_inertia_world, &H.iiw[9 * i], 36);
       memcpy(s.aabb_lower, &H.aabb_lo[3 * i], 12); memcpy(s.aabb_upper, &H.aabb_hi[3 * i], 12);
       s.aabb_needs_update = (H.flags[i] & BF_AABB_DIRTY) ? 1 : 0;
+      s.sleep_state = H.sleep_state[i];
+      s.time_last_sleepy = H.sleep_tls[i];
     }
   }
   return CHP_OK;
@@ -1939,7 +2004,17 @@ int chp_world_stats(chp_ctx* ctx, uint32_t id, chp_step_stats* out) {
   out->sum_pairs = (uint32_t)st[C_SUM_PAIRS];
   out->sum_rows = (uint32_t)st[C_SUM_ROWS];
   out->sum_contacts = (uint32_t)st[C_SUM_CONTACTS];
-  return CHP_OK;
+  out->reserved = 0;
+  return ctx->precision == CHP_PRECISION_F64 ? sleep_stats_t<double>(ctx, id, out) : sleep_stats_t<float>(ctx, id, out);
+}
+
+int chp_world_set_time(chp_ctx* ctx, uint32_t id, double time) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!std::isfinite(time)) return fail(ctx, CHP_E_INVALID, "world time must be finite");
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? set_time_t<double>(ctx, id, time) : set_time_t<float>(ctx, id, time);
 }
 
 

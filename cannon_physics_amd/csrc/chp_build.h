@@ -92,6 +92,10 @@ template <typename R> struct HostWorld {
   std::vector<R> mass, inv_mass, brad;
   std::vector<float> inertia;
   std::vector<int> type, flags, group, mask, material, shape_first, shape_count, order;
+  // sleep: state (CHP_SLEEP_*), Body.timeLastSleepy (a Dart double in every mode), limits
+  std::vector<int> sleep_state;
+  std::vector<double> sleep_tls, sleep_time;
+  std::vector<R> sleep_speed;
   // shape instances
   std::vector<int> si_type, si_hull, si_hf, si_response, si_group, si_mask, si_material;
   std::vector<float> si_off, si_ori, si_half;
@@ -114,6 +118,7 @@ template <typename R> struct HostWorld {
   // world params
   chp_world_desc wd;
   int sap_axis = 0;
+  double time = 0;   // World.time (kept on the host when the slab is rebuilt)
 };
 
 // Shape.calculateWorldAABB for one body-shape instance, in FP64 (box.dart:149-192,
@@ -384,7 +389,13 @@ template <typename R> struct Builder {
       if (d.type >= 0) type = d.type;
       bodies.push_back(b);
       W.type.push_back(type);
-      int fl = (d.collision_response ? BF_RESPONSE : 0) | (d.is_trigger ? BF_TRIGGER : 0) | (b.dirty ? BF_AABB_DIRTY : 0);
+      int fl = (d.collision_response ? BF_RESPONSE : 0) | (d.is_trigger ? BF_TRIGGER : 0) | (b.dirty ? BF_AABB_DIRTY : 0) |
+               (d.allow_sleep ? BF_ALLOW_SLEEP : 0);
+      if (d.sleep_state < CHP_SLEEP_AWAKE || d.sleep_state > CHP_SLEEP_SLEEPING) fail("bad body sleep state");
+      W.sleep_state.push_back(d.sleep_state);
+      W.sleep_tls.push_back(0.0);   // World.addBody: timeLastSleepy = world.time (0 at construction)
+      W.sleep_speed.push_back((R)d.sleep_speed_limit);
+      W.sleep_time.push_back(d.sleep_time_limit);
       if (b.inv_inertia.x == b.inv_inertia.y && b.inv_inertia.y == b.inv_inertia.z) fl |= BF_ISOTROPIC;
       W.flags.push_back(fl);
       W.group.push_back(d.collision_filter_group);

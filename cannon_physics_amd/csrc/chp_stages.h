@@ -136,6 +136,16 @@ template <typename R> __device__ void body_aabb(const DevWorld<R>& W, int i) {
 // World.internalStep before the broadphase for body i (world_class.dart:459-471): gravity into
 // the effective force, the solve masses, cleared velocity deltas, the SAP AABB refresh
 // SAP: the world may be a SAP world (its dirty AABBs are refreshed); else the code is left out
+// a body the broadphase treats as fixed: static, or asleep (broadphase.dart:54-60)
+template <typename R> __device__ __forceinline__ bool bp_passive(const DevWorld<R>& W, int i) {
+  return W.type[i] == CHP_BODY_STATIC || W.sleep_state[i] == CHP_SLEEP_SLEEPING;
+}
+// a body whose solve mass is zero: kinematic, or asleep (Body.updateSolveMassProperties,
+// rigid_body.dart:302-314)
+template <typename R> __device__ __forceinline__ bool solve_massless(const DevWorld<R>& W, int i) {
+  return W.type[i] == CHP_BODY_KINEMATIC || W.sleep_state[i] == CHP_SLEEP_SLEEPING;
+}
+
 template <typename R, bool SAP = true> __device__ __forceinline__ void prestep_body(const DevWorld<R>& W, int i) {
   int ty = W.type[i];
   f3 f = F3<R>(W.force, i);
@@ -144,7 +154,7 @@ template <typename R, bool SAP = true> __device__ __forceinline__ void prestep_b
     f = set3<R>(ld<R>(f.x) + m * (R)W.gravity[0], ld<R>(f.y) + m * (R)W.gravity[1], ld<R>(f.z) + m * (R)W.gravity[2]);
   }
   S3<R>(W.feff, i, f);
-  bool kin = ty == CHP_BODY_KINEMATIC;
+  const bool kin = solve_massless(W, i);
   W.ims[i] = kin ? (R)0 : W.inv_mass[i];
   for (int k = 0; k < 9; k++) W.iiws[9 * i + k] = kin ? 0.f : W.iiw[9 * i + k];
   S3<R>(W.vl, i, mk3(0.f, 0.f, 0.f));
@@ -165,7 +175,7 @@ template <typename R, bool SAP = true> __device__ void st_prestep(const DevWorld
 // (:77-86), FP64 predicate
 template <typename R> __device__ __forceinline__ bool bp_need(const DevWorld<R>& W, int a, int b) {
   if ((W.group[a] & W.mask[b]) == 0 || (W.group[b] & W.mask[a]) == 0) return false;
-  if (W.type[a] == CHP_BODY_STATIC && W.type[b] == CHP_BODY_STATIC) return false;
+  if (bp_passive(W, a) && bp_passive(W, b)) return false;
   return true;
 }
 __device__ __forceinline__ bool bp_sphere_at(f3 pa, double ra, f3 pb, double rb) {
@@ -196,7 +206,7 @@ template <typename R> struct BpLds {
   float* pos;  // 3 per body
   R* brad;
   int* gm;     // group, mask per body
-  int* stat;   // 1: static
+  int* stat;   // 1: static or asleep
   static __host__ __device__ size_t bytes(int n) {
     return (size_t)bp_chunks(n) * sizeof(unsigned long long) + (size_t)n * (3 * sizeof(float) + sizeof(R) + 3 * sizeof(int));
   }
@@ -234,7 +244,7 @@ template <typename R, bool LDS> __device__ bool st_bp_naive(const DevWorld<R>& W
       L.brad[i] = W.brad[i];
       L.gm[2 * i] = W.group[i];
       L.gm[2 * i + 1] = W.mask[i];
-      L.stat[i] = W.type[i] == CHP_BODY_STATIC;
+      L.stat[i] = bp_passive(W, i);
     }
     __syncthreads();
   }
@@ -381,7 +391,7 @@ template <typename R> __device__ __forceinline__ bool bp_filtered(const DevWorld
 template <typename R> __device__ __forceinline__ void sap_record(const DevWorld<R>& W, int k, int b) {
   const f3 p = F3<R>(W.pos, b);
   ((float4*)W.sbp)[k] = make_float4(p.x, p.y, p.z, __int_as_float(b));
-  ((int4*)W.sgm)[k] = make_int4(W.group[b], W.mask[b], W.type[b] == CHP_BODY_STATIC ? 1 : 0, 0);
+  ((int4*)W.sgm)[k] = make_int4(W.group[b], W.mask[b], bp_passive(W, b) ? 1 : 0, 0);
   W.sbr[k] = W.brad[b];
 }
 
@@ -798,7 +808,7 @@ template <typename R> __device__ __forceinline__ R mulvec(f3 s, f3 r, f3 gs, f3 
 // The body state the row builders read (Equation.bi / bj), loaded once per contact or joint and
 // shared by its rows: the row stores do not alias it, but the compiler cannot know that.
 template <typename R> struct RowBody {
-  int i, fl;            // index; with GS_DYN when dynamic
+  int i, fl;            // index; with GS_DYN when dynamic and awake (its vlambda moves)
   f3 p, v, w, f, t;     // position, velocity, angular velocity, effective force, torque
   R im;                 // invMassSolve
   float I[9];           // invInertiaWorldSolve
@@ -806,7 +816,7 @@ template <typename R> struct RowBody {
 template <typename R> __device__ __forceinline__ RowBody<R> row_body(const DevWorld<R>& W, int i) {
   RowBody<R> o;
   o.i = i;
-  o.fl = i | (W.type[i] == CHP_BODY_DYNAMIC ? (int)GS_DYN : 0);
+  o.fl = i | (W.type[i] == CHP_BODY_DYNAMIC && W.sleep_state[i] != CHP_SLEEP_SLEEPING ? (int)GS_DYN : 0);
   o.p = F3<R>(W.pos, i);
   o.v = F3<R>(W.vel, i);
   o.w = F3<R>(W.avel, i);
@@ -1080,7 +1090,7 @@ template <typename R> __device__ void st_equations(const DevWorld<R>& W, R h, ch
 template <typename R> __device__ void st_solve_rows(const DevWorld<R>& W, const chp_equation* eq, int n, R h) {
   for (int i = threadIdx.x; i < W.nb; i += blockDim.x) {
     S3<R>(W.feff, i, F3<R>(W.force, i));
-    bool kin = W.type[i] == CHP_BODY_KINEMATIC;
+    const bool kin = solve_massless(W, i);
     W.ims[i] = kin ? (R)0 : W.inv_mass[i];
     for (int k = 0; k < 9; k++) W.iiws[9 * i + k] = kin ? 0.f : W.iiw[9 * i + k];
     S3<R>(W.vl, i, mk3(0.f, 0.f, 0.f));
@@ -1476,10 +1486,13 @@ __device__ __forceinline__ R gs_row(const GsRow<R>& cr, R clam, bool v, int ks, 
 // are loaded when they are reached.  A row recurs after L steps: when L < D a ring slot may hold
 // a row whose lambda changes before the slot is used, and the new lambda is forwarded to it.
 // lam / adl have one dummy slot at row_cap for idle lanes.
+#ifndef CHP_GS_DEPTH_F32
+#define CHP_GS_DEPTH_F32 2   // ring depth D of the FP32 sweep (2 measured a few per cent faster than 4)
+#endif
 template <typename R, typename G, bool LDS>
 __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* shd, int* sh, float* vw, int* ls_lds,
                                       bool keep) {
-  constexpr int D = sizeof(R) == 4 ? 4 : 2;
+  constexpr int D = sizeof(R) == 4 ? CHP_GS_DEPTH_F32 : 2;
   const int E = uni(W.counters[C_ROWS]), L = uni(W.counters[C_LEVELS]), iters = uni(W.iterations);
   const int nb = W.nb, tid = g.tid, bd = g.n, dummy = W.row_cap;
   const R tol2 = W.tol2;
@@ -1505,18 +1518,7 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
     // a step is one group-wide slice of a level: a level of c rows takes ceil(c / bd) steps
     // (its rows are independent, so slicing it keeps the sequential sweep's values)
     int pl = 0, pj = 0;   // level and slice of the next step to load
-    // the step's row of this thread (k, valid) and the next step's (level, slice)
-    auto next = [&](int& k, bool& valid) {
-      const int a0 = ls[pl], a1 = ls[pl + 1];
-      k = a0 + pj * bd + tid;
-      valid = k < a1;
-      if (a0 + (pj + 1) * bd < a1) {
-        pj++;
-      } else {
-        pj = 0;
-        pl = (pl + 1 == L) ? 0 : pl + 1;
-      }
-    };
+    int ps = 0;           // its step within the sweep
     // steps per sweep
     int lp = 0;
     for (int l = tid; l < L; l += bd) {
@@ -1524,68 +1526,105 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
       lp += c > bd ? (c + bd - 1) / bd : 1;
     }
     const int Lp = uni((int)g.sum((double)lp, shd));
-#pragma unroll
-    for (int s = 0; s < D; s++) {
-      int k;
-      next(k, qv[s]);
-      qk[s] = qv[s] ? k : 0;
-      gs_load<R>(gsr, qk[s], q[s]);
-      ql[s] = lam[qk[s]];
-      qf[s] = (R)0;
-      qo[s] = false;
-      // keep the slots' loads in slot order, as the steady state issues them: the wait before
-      // the first step of the loop then allows the later slots to stay in flight
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    int step = 0;   // of the sweep
-    double part = 0.0;
-    for (;;) {
-#pragma unroll
-      for (int s = 0; s < D; s++) {
-        const int k = qk[s];
-        const bool v = qv[s];
-        const R lnew = gs_row<R>(q[s], qo[s] ? qf[s] : ql[s], v, v ? k : dummy, vw, lam, adl, part);
-        // forwarding touches only qf / qo: a select on ql would wait for loads still in flight
-#pragma unroll
-        for (int s2 = 0; s2 < D; s2++)
-          if (s2 != s && v && qv[s2] && qk[s2] == k) {
-            qf[s2] = lnew;
-            qo[s2] = true;
-          }
-        {  // the slot's next row (D steps on)
-          int kn;
-          bool vn;
-          next(kn, vn);
-          int kc = vn ? kn : 0;
-          gs_load<R>(gsr, kc, q[s]);
-          ql[s] = lam[kc];
-          qf[s] = lnew;
-          qo[s] = v && vn && kc == k;
-          qk[s] = kc;
-          qv[s] = vn;
-        }
-        if constexpr (LDS) g.lds_sync();   // the next step reads what this one wrote
-        else g.sync();
-        if (++step == Lp) {   // end of a sweep: GSSolver's early-exit test
-          step = 0;
-          double Dsum = g.sum(part, shd);
-          int dec = gs_decide<R>(Dsum, E, tol2);
-          if (uni(dec) < 0) {
-            g.sync();   // every |dlambda| of the sweep is stored
-            int vv = 0;
-            if (tid == 0) {
-              R tot = 0;
-              for (int r = 0; r < E; r++) tot += adl[W.rpos[r]];
-              vv = tot * tot < tol2 ? 1 : 0;
-            }
-            dec = g.bcast0(vv, sh);
-          }
-          part = 0.0;
-          if (uni(dec) || ++it == iters) goto swept;
+    // A sweep is padded with idle steps (no row on any thread) to a multiple of the ring depth,
+    // so that a sweep always ends after the ring's last slot: the loop then has one exit test
+    // per turn, and a loop with one exit keeps the wait counts exact (exits after each slot make
+    // the compiler merge the paths and drain every prefetch at the loop head).
+    const int Lpad = (Lp + D - 1) / D * D;
+    // the step's row of this thread (k, valid), then the cursor moves to the next step
+    auto next = [&](int& k, bool& valid) {
+      if (ps >= Lp) {
+        k = 0;
+        valid = false;
+      } else {
+        const int a0 = ls[pl], a1 = ls[pl + 1];
+        k = a0 + pj * bd + tid;
+        valid = k < a1;
+        if (a0 + (pj + 1) * bd < a1) {
+          pj++;
+        } else {
+          pj = 0;
+          pl++;
         }
       }
+      if (++ps == Lpad) ps = pl = pj = 0;
+    };
+    const int* rpos = W.rpos;
+    // The sweeps run in an inner loop whose only memory traffic is the ring's loads and the
+    // lambda stores.  The rare undecided early-exit test reads the stored |dlambda| outside
+    // it, and the ring then starts again at the first step of the next sweep.
+    for (;;) {
+      ps = pl = pj = 0;
+#pragma unroll
+      for (int s = 0; s < D; s++) {
+        int k;
+        next(k, qv[s]);
+        qk[s] = qv[s] ? k : 0;
+        gs_load<R>(gsr, qk[s], q[s]);
+        ql[s] = lam[qk[s]];
+        qf[s] = (R)0;
+        qo[s] = false;
+        // keep the slots' loads in slot order, as the steady state issues them: the wait before
+        // the first step of the loop then allows the later slots to stay in flight
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      int step = 0;   // of the sweep
+      int dec = 0;    // the sweep-end decision: 1 stop, -1 undecided
+      double part = 0.0;
+      for (;;) {
+#pragma unroll
+        for (int s = 0; s < D; s++) {
+          const int k = qk[s];
+          const bool v = qv[s];
+          const R lnew = gs_row<R>(q[s], qo[s] ? qf[s] : ql[s], v, v ? k : dummy, vw, lam, adl, part);
+          // forwarding touches only qf / qo: a select on ql would wait for loads still in flight
+#pragma unroll
+          for (int s2 = 0; s2 < D; s2++)
+            if (s2 != s && v && qv[s2] && qk[s2] == k) {
+              qf[s2] = lnew;
+              qo[s2] = true;
+            }
+          {  // the slot's next row (D steps on)
+            int kn;
+            bool vn;
+            next(kn, vn);
+            int kc = vn ? kn : 0;
+            gs_load<R>(gsr, kc, q[s]);
+            ql[s] = lam[kc];
+            qf[s] = lnew;
+            qo[s] = v && vn && kc == k;
+            qk[s] = kc;
+            qv[s] = vn;
+          }
+          if constexpr (LDS) g.lds_sync();   // the next step reads what this one wrote
+          else g.sync();
+        }
+        step += D;
+        if (step == Lpad) {   // end of a sweep: GSSolver's early-exit test
+          step = 0;
+          const double Dsum = g.sum(part, shd);
+          part = 0.0;
+          dec = uni(gs_decide<R>(Dsum, E, tol2));
+          if (dec != 0) break;
+          if (++it == iters) {
+            dec = 1;
+            break;
+          }
+        }
+      }
+      if (dec > 0) break;
+      // undecided: the sequential sum in solver order decides (every |dlambda| of the sweep is
+      // stored)
+      g.sync();
+      int vv = 0;
+      if (tid == 0) {
+        R tot = 0;
+        for (int r = 0; r < E; r++) tot += adl[rpos[r]];
+        vv = tot * tot < tol2 ? 1 : 0;
+      }
+      if (uni(g.bcast0(vv, sh)) || ++it == iters) break;
+      // next sweep: the ring starts again at its first step
     }
-  swept:
     g.sync();
     // Equation.multiplier and the capture read lambda from the row records (keep)
     if (keep)
@@ -1602,7 +1641,60 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
   g.sync();
 }
 
-// body i's velocity update, damping and integration; false when its pose turned non-finite
+// ---- sleep ------------------------------------------------------------------------------------
+// the wake-up marks of one contact side (world_class.dart:562-590): sleeping dynamic body a,
+// allowed to sleep, touched by awake non-static body b whose speed^2 (|v|^2 + |w|^2) is at least
+// twice b's sleep speed limit squared
+template <typename R> __device__ __forceinline__ bool wake_mark(const DevWorld<R>& W, int a, int b) {
+  if (!(W.flags[a] & BF_ALLOW_SLEEP) || W.type[a] != CHP_BODY_DYNAMIC || W.sleep_state[a] != CHP_SLEEP_SLEEPING)
+    return false;
+  if (W.sleep_state[b] != CHP_SLEEP_AWAKE || W.type[b] == CHP_BODY_STATIC) return false;
+  const R s2 = len2<R>(F3<R>(W.vel, b)) + len2<R>(F3<R>(W.avel, b));
+  const R lim = W.sleep_speed[b];
+  return s2 >= lim * lim * (R)2;
+}
+// Body.wakeUpAfterNarrowphase over the step's contacts, then the wake-up pass after them
+// (world_class.dart:617-624): a woken body is awake and gets its solve mass back before the
+// solver reads it.  The marks read only the states from before the pass, so the contacts' order
+// does not matter.  One workgroup per world.
+template <typename R> __device__ void st_wake(const DevWorld<R>& W) {
+  const int nc = W.counters[C_STAGING];
+  for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    const int bi = W.contacts[k].bi, bj = W.contacts[k].bj;
+    if (wake_mark(W, bi, bj)) atomicOr(&W.flags[bi], (int)BF_WAKE);
+    if (wake_mark(W, bj, bi)) atomicOr(&W.flags[bj], (int)BF_WAKE);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < W.nb; i += blockDim.x) {
+    const int fl = W.flags[i];
+    if (!(fl & BF_WAKE)) continue;
+    W.flags[i] = fl & ~BF_WAKE;
+    W.sleep_state[i] = CHP_SLEEP_AWAKE;
+    W.ims[i] = W.inv_mass[i];   // a dynamic body: Body.updateSolveMassProperties
+    for (int k = 0; k < 9; k++) W.iiws[9 * i + k] = W.iiw[9 * i + k];
+  }
+}
+// Body.sleepTick (rigid_body.dart:280-300) at World.time before this step's increment, on the
+// velocities the step leaves; Body.sleep() zeroes them
+template <typename R> __device__ __forceinline__ void sleep_tick(const DevWorld<R>& W, int i, f3& v, f3& w) {
+  const int s = W.sleep_state[i];
+  const R s2 = len2<R>(v) + len2<R>(w);
+  const R lim = W.sleep_speed[i], l2 = lim * lim;
+  const double t = W.wtime[0];
+  if (s == CHP_SLEEP_AWAKE && s2 < l2) {
+    W.sleep_state[i] = CHP_SLEEP_SLEEPY;
+    W.sleep_tls[i] = t;
+  } else if (s == CHP_SLEEP_SLEEPY && s2 > l2) {
+    W.sleep_state[i] = CHP_SLEEP_AWAKE;
+  } else if (s == CHP_SLEEP_SLEEPY && t - W.sleep_tls[i] > W.sleep_time[i]) {
+    W.sleep_state[i] = CHP_SLEEP_SLEEPING;
+    v = mk3(0.f, 0.f, 0.f);
+    w = mk3(0.f, 0.f, 0.f);
+  }
+}
+
+// body i's velocity update, damping, integration and sleep timer; false when its pose turned
+// non-finite
 template <typename R> __device__ bool post_body(const DevWorld<R>& W, int i, R dt, int E, bool qn) {
   bool ok = true;
   f3 v = F3<R>(W.vel, i), w = F3<R>(W.avel, i);
@@ -1616,7 +1708,7 @@ template <typename R> __device__ bool post_body(const DevWorld<R>& W, int i, R d
     v = scale3<R>(v, W.ldamp[i]);
     w = scale3<R>(w, W.adamp[i]);
   }
-  if (ty == CHP_BODY_DYNAMIC || ty == CHP_BODY_KINEMATIC) {
+  if ((ty == CHP_BODY_DYNAMIC || ty == CHP_BODY_KINEMATIC) && W.sleep_state[i] != CHP_SLEEP_SLEEPING) {
     f3 f = F3<R>(W.feff, i), tq = F3<R>(W.torque, i);
     R iMdt = W.inv_mass[i] * dt;
     v = set3<R>(ld<R>(v.x) + ld<R>(f.x) * iMdt * ld<R>(lf.x), ld<R>(v.y) + ld<R>(f.y) * iMdt * ld<R>(lf.y),
@@ -1658,6 +1750,7 @@ template <typename R> __device__ bool post_body(const DevWorld<R>& W, int i, R d
     if (!(W.flags[i] & BF_ISOTROPIC)) inertia_world<R>(q, F3<R>(W.inv_inertia, i), W.iiw + 9 * i);
     ok = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(q.w);
   }
+  if (W.allow_sleep && (W.flags[i] & BF_ALLOW_SLEEP)) sleep_tick(W, i, v, w);
   S3<R>(W.vel, i, v);
   S3<R>(W.avel, i, w);
   S3<R>(W.force, i, mk3(0.f, 0.f, 0.f));   // World.clearForces
@@ -1671,13 +1764,18 @@ template <typename R> __device__ __forceinline__ bool post_qn(const DevWorld<R>&
 
 // ---- velocities, damping, integration ---------------------------------------------------------
 // gs_solver.dart:157-168, world_class.dart:647-674, Body.integrate (rigid_body.dart:627-680)
-template <typename R> __device__ void st_post(const DevWorld<R>& W, R dt) {
-  const int E = W.counters[C_ROWS], step = W.counters[C_STEP];
+// the end of a step: the step counter, World.time += dt (world_class.dart:397-399)
+template <typename R> __device__ __forceinline__ void step_done(const DevWorld<R>& W, double dt) {
+  W.counters[C_STEP] += 1;
+  W.wtime[0] += dt;
+}
+template <typename R> __device__ void st_post(const DevWorld<R>& W, double dt) {
+  const int E = W.counters[C_ROWS];
   const bool qn = post_qn(W);
   for (int i = threadIdx.x; i < W.nb; i += blockDim.x)
-    if (!post_body(W, i, dt, E, qn)) ws_fail(W.counters, WS_NONFINITE);
+    if (!post_body(W, i, (R)dt, E, qn)) ws_fail(W.counters, WS_NONFINITE);
   __syncthreads();
-  if (threadIdx.x == 0) W.counters[C_STEP] = step + 1;
+  if (threadIdx.x == 0) step_done(W, dt);
   __syncthreads();
 }
 

@@ -30,6 +30,8 @@ enum : int {
   BF_TRIGGER = 2,        // Body.isTrigger
   BF_AABB_DIRTY = 4,     // Body.aabbNeedsUpdate
   BF_ISOTROPIC = 8,      // invInertia.x == .y == .z (updateInertiaWorld skips)
+  BF_ALLOW_SLEEP = 16,   // Body.allowSleep
+  BF_WAKE = 32,          // Body.wakeUpAfterNarrowphase
 };
 
 enum : int { WS_OK = 0, WS_PAIRS = 1, WS_CONTACTS = 2, WS_ROWS = 4, WS_NONFINITE = 8, WS_STAGING = 16 };
@@ -66,7 +68,7 @@ template <typename R> struct DevWorld {
   int pair_cap, contact_cap, row_cap, nmat;
   // parameters
   int broadphase, sap_axis, iterations, quat_skip;
-  int quat_fast, has_filtered_constraints, stepnumber, pad0;
+  int quat_fast, has_filtered_constraints, stepnumber, allow_sleep;   // allow_sleep: World.allowSleep
   float gravity[3];
   float pad1;
   R fric_grav_len, tol2;
@@ -76,6 +78,11 @@ template <typename R> struct DevWorld {
   CHP_G(R) mass, inv_mass, brad, ldamp, adamp;
   CHP_G(int) type, flags, group, mask, material, shape_first, shape_count;
   CHP_G(int) order;  // SAPBroadphase.axisList (body indices), persistent across steps
+  // sleep (rigid_body.dart:263-314): state per body, Body.timeLastSleepy, the limits; the
+  // world's time (World.time, the sleep timer's clock) in wtime[0]
+  CHP_G(int) sleep_state;
+  CHP_G(double) sleep_tls, sleep_time, wtime;
+  CHP_G(R) sleep_speed;
   // body-shape instances
   CHP_G(int) si_type, si_hull, si_hf, si_response, si_group, si_mask, si_material;
   CHP_G(float) si_off, si_ori, si_half;

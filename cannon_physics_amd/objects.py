@@ -209,7 +209,7 @@ class Body:
                  angularVelocity=None, shape=None, material=None, linearDamping=0.01,
                  angularDamping=0.01, type=None, collisionFilterGroup=1, collisionFilterMask=-1,
                  collisionResponse=True, isTrigger=False, fixedRotation=False, linearFactor=None,
-                 angularFactor=None, allowSleep=True):
+                 angularFactor=None, allowSleep=True, sleepSpeedLimit=0.1, sleepTimeLimit=1.0):
         self.mass = float(mass)
         self.material = material
         self.linearDamping = float(linearDamping)
@@ -221,6 +221,9 @@ class Body:
         self.isTrigger = bool(isTrigger)
         self.fixedRotation = bool(fixedRotation)
         self.allowSleep = allowSleep
+        self.sleepSpeedLimit = float(sleepSpeedLimit)
+        self.sleepTimeLimit = float(sleepTimeLimit)
+        self.sleepState = A.SLEEP_AWAKE
         self._ctor_position = _v3(position)
         self._ctor_quaternion = _q4(quaternion)
         self.position = Vector3(*self._ctor_position)
@@ -237,6 +240,20 @@ class Body:
         self.index = -1
         if shape is not None:
             self.addShape(shape)
+
+    def sleep(self):
+        """Body.sleep() (rigid_body.dart:272-278), before the world's first step."""
+        if self.world is not None:
+            self.world._require_host("Body.sleep")
+        self.sleepState = A.SLEEP_SLEEPING
+        self.velocity.setValues(0, 0, 0)
+        self.angularVelocity.setValues(0, 0, 0)
+
+    def wakeUp(self):
+        """Body.wakeUp() (rigid_body.dart:262-270), before the world's first step."""
+        if self.world is not None:
+            self.world._require_host("Body.wakeUp")
+        self.sleepState = A.SLEEP_AWAKE
 
     def addShape(self, shape, offset=None, orientation=None):
         if (self.position.tolist() != [float(np.float32(c)) for c in self._ctor_position]
@@ -440,6 +457,7 @@ def compile_scene(world):
     wd.contact_relaxation = cm.contactEquationRelaxation
     wd.friction_stiffness = cm.frictionEquationStiffness
     wd.friction_relaxation = cm.frictionEquationRelaxation
+    wd.allow_sleep = int(bool(world.allowSleep))
 
     materials, mat_index = [], {}
 
@@ -517,6 +535,10 @@ def compile_scene(world):
         d.angular_velocity[:] = b.angularVelocity.tolist()
         d.linear_factor[:] = b.linearFactor.tolist()
         d.angular_factor[:] = b.angularFactor.tolist()
+        d.allow_sleep = int(bool(b.allowSleep))
+        d.sleep_state = int(b.sleepState)
+        d.sleep_speed_limit = b.sleepSpeedLimit
+        d.sleep_time_limit = b.sleepTimeLimit
         d.first_shape = len(body_shapes)
         d.num_shapes = len(b.shapes)
         d.ctor_shape = int(b._ctor_shape)

@@ -7,6 +7,8 @@
   c5_terrain_world  one serving request: spheres/boxes/convex hulls over a value-noise terrain
   feature_matrix_world  every body, shape, material and world option the recipe accepts, in
                     one world (parity coverage, not a benchmark workload)
+  sleep_world       World.allowSleep: bodies settling into sleep, pre-slept bodies woken (or
+                    not) by faster and slower bodies (parity coverage)
 
 Positions and sizes are drawn in double and rounded to FP32 on store, as the reference's
 Vector3 does.
@@ -370,4 +372,55 @@ def prism_world(seed=9, segments=20):
         w.addBody(Body(mass=1.0, position=(x + 0.1, y, z), shape=hull))
         y += 0.25 + 0.2 + 0.01
         w.addBody(Body(mass=0.5, position=(x - 0.1, y, z + 0.1), shape=Sphere(0.2)))
+    return w
+
+
+def sleep_world(seed=21, broadphase="naive"):
+    """World({allowSleep: true}) (world_class.dart:686-699, rigid_body.dart:263-314):
+      - boxes and spheres dropped onto the plane with their own sleep speed / time limits: they
+        go sleepy, some wake again on a bounce, then fall asleep;
+      - three bodies put to sleep before the first step (Body.sleep()) resting on the plane: a
+        fast sphere rolls into the first (speed^2 >= 2 limit^2: it wakes), a slow box slides into
+        the second (it stays asleep and weightless in the solve), a box lands on the third;
+      - a sleeping box on a sleeping box (no broadphase pair between them);
+      - a body with allowSleep = false and a kinematic body gliding at constant speed."""
+    rng = np.random.default_rng(1_000_003 * 21 + seed)
+    bp = SAPBroadphase() if broadphase == "sap" else NaiveBroadphase()
+    w = World(gravity=(0, -9.82, 0), broadphase=bp, solver=GSSolver(iterations=10), allowSleep=True)
+    w.addBody(ground_plane())
+    for k in range(12):
+        p = (-3.0 + 0.55 * k, rng.uniform(0.5, 1.2), rng.uniform(-0.3, 0.3))
+        lim, tl = rng.uniform(0.15, 0.5), rng.uniform(0.2, 0.6)
+        if k % 2:
+            b = Body(mass=1.0, position=p, shape=Sphere(0.22), sleepSpeedLimit=lim, sleepTimeLimit=tl)
+        else:
+            b = Body(mass=1.0, position=p, quaternion=shoemake_quaternion(rng), shape=Box((0.2, 0.2, 0.2)),
+                     sleepSpeedLimit=lim, sleepTimeLimit=tl)
+        w.addBody(b)
+    sleepers = []
+    for k, x in enumerate((-2.0, 0.0, 2.0)):
+        b = Body(mass=1.0, position=(x, 0.3, 2.0), shape=Sphere(0.3), sleepSpeedLimit=0.3)
+        b.sleep()
+        w.addBody(b)
+        sleepers.append(b)
+    # fast: |v|^2 + |w|^2 >= 2 * 0.4^2 (the moving body's own limit decides); slow: a sliding
+    # box, |v|^2 = 0.04 < 2 * 0.3^2
+    w.addBody(Body(mass=1.0, position=(-3.2, 0.3, 2.0), velocity=(1.5, 0.0, 0.0), shape=Sphere(0.3),
+                   sleepSpeedLimit=0.4))
+    w.addBody(Body(mass=1.0, position=(-0.5497, 0.25, 2.0), velocity=(0.2, 0.0, 0.0), shape=Box((0.25, 0.25, 0.25)),
+                   sleepSpeedLimit=0.3, sleepTimeLimit=5.0))
+    w.addBody(Body(mass=1.0, position=(2.0, 1.4, 2.0), quaternion=shoemake_quaternion(rng),
+                   shape=Box((0.25, 0.25, 0.25))))
+    # a sleeping stack of two boxes
+    low = Body(mass=1.0, position=(0.0, 0.25, -2.0), shape=Box((0.25, 0.25, 0.25)))
+    top = Body(mass=1.0, position=(0.0, 0.75, -2.0), shape=Box((0.25, 0.25, 0.25)))
+    low.sleep()
+    top.sleep()
+    w.addBody(low)
+    w.addBody(top)
+    w.addBody(Body(mass=1.0, position=(1.0, 0.8, -1.0), shape=Sphere(0.25), allowSleep=False))
+    w.addBody(Body(mass=2.0, type=2, position=(-3.0, 0.2, -1.0), velocity=(0.5, 0.0, 0.0),
+                   shape=Box((0.2, 0.2, 0.2))))
+    if broadphase == "sap":
+        bp.autoDetectAxis()
     return w

@@ -124,7 +124,11 @@ class Context:
     def stats(self, wid):
         s = A.StepStats()
         self.check(self.lib.chp_world_stats(self.ptr, wid, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in A.StepStats._fields_}
+        return {k: getattr(s, k) for k, _ in A.StepStats._fields_ if k != "reserved"}
+
+    def set_time(self, wid, t):
+        """World.time of a world (the clock of the sleep timer)"""
+        self.check(self.lib.chp_world_set_time(self.ptr, wid, float(t)))
 
     def set_capacity(self, wid, max_pairs=0, max_contacts=0, max_rows=0):
         self.check(self.lib.chp_world_set_capacity(self.ptr, wid, max_pairs, max_contacts, max_rows))
@@ -227,14 +231,13 @@ class _BodyView:
 
 class World:
     """World({gravity, frictionGravity, allowSleep, broadphase, solver, quatNormalizeFast,
-    quatNormalizeSkip}) (world_class.dart:135-162).  allowSleep must stay false: sleep is not
-    part of the device pipeline."""
+    quatNormalizeSkip}) (world_class.dart:135-162).  With allowSleep the device runs the sleep
+    timer of every body after each step (world_class.dart:686-699)."""
 
     def __init__(self, gravity=None, frictionGravity=None, allowSleep=False, broadphase=None,
                  solver=None, quatNormalizeFast=False, quatNormalizeSkip=0, context=None,
                  precision=A.PRECISION_F32):
-        if allowSleep:
-            raise NotImplementedError("allowSleep=True is not supported by the device pipeline")
+        self.allowSleep = bool(allowSleep)
         self.gravity = Vector3(*_v3(gravity))
         self.frictionGravity = None if frictionGravity is None else Vector3(*_v3(frictionGravity))
         self.broadphase = broadphase if broadphase is not None else NaiveBroadphase()
@@ -298,20 +301,23 @@ class World:
         """Fixed stepping (world_class.dart:392-399)."""
         if timeSinceLastCalled is not None:
             raise NotImplementedError("interpolated stepping is not part of the device pipeline")
-        wid = self._ensure_device()
-        self.dt = dt
-        self.context.step([wid], dt, 1)
-        self.time += dt
-        self.stepnumber += 1
-        self._state = None
+        self.step_n(dt, 1)
 
     def step_n(self, dt, n):
+        """n fixed steps; World.time is the device's sum of the steps' dt."""
         wid = self._ensure_device()
         self.dt = dt
         self.context.step([wid], dt, n)
-        self.time += dt * n
+        self.time = self.context.stats(wid)["time"]
         self.stepnumber += n
         self._state = None
+
+    @property
+    def hasActiveBodies(self):
+        """World.hasActiveBodies after the last step: false when every body sleeps."""
+        if not self.allowSleep or self._wid is None:
+            return True
+        return bool(np.any(self.state()["sleep_state"] != A.SLEEP_SLEEPING))
 
     def state(self):
         """Current per-body state as float32 numpy arrays (downloaded once per step)."""

@@ -45,6 +45,8 @@ enum { CHP_BODY_DYNAMIC = 0, CHP_BODY_STATIC = 1, CHP_BODY_KINEMATIC = 2 };
 enum { CHP_BROADPHASE_NAIVE = 0, CHP_BROADPHASE_SAP = 1 };
 enum { CHP_CONSTRAINT_POINT_TO_POINT = 0, CHP_CONSTRAINT_HINGE = 1, CHP_CONSTRAINT_LOCK = 2,
        CHP_CONSTRAINT_DISTANCE = 3, CHP_CONSTRAINT_CONE_TWIST = 4 };
+/* BodySleepStates, lib/objects/rigid_body.dart */
+enum { CHP_SLEEP_AWAKE = 0, CHP_SLEEP_SLEEPY = 1, CHP_SLEEP_SLEEPING = 2 };
 /* Equation subclasses, lib/equations/ */
 enum { CHP_ROW_FRICTION = 0, CHP_ROW_CONTACT = 1, CHP_ROW_ROTATIONAL = 2, CHP_ROW_CONE = 3,
        CHP_ROW_MOTOR = 4 };
@@ -67,6 +69,8 @@ typedef struct chp_world_desc {          /* World({...}) world_class.dart:135-16
   /* World.defaultContactMaterial, world_class.dart:155-158 + contact_material.dart:37-76 */
   double friction, restitution;
   double contact_stiffness, contact_relaxation, friction_stiffness, friction_relaxation;
+  int32_t allow_sleep;                   /* World.allowSleep (default 0) */
+  int32_t reserved;
 } chp_world_desc;
 
 typedef struct chp_material_desc {       /* Material, material.dart:19-25 */
@@ -130,6 +134,10 @@ typedef struct chp_body_desc {           /* Body({...}) rigid_body.dart:27-86 */
   float quaternion_after[4];
   int32_t set_position;                  /* position assigned after construction */
   float position_after[3];
+  int32_t allow_sleep;                   /* Body.allowSleep (the reference's default is 1) */
+  int32_t sleep_state;                   /* CHP_SLEEP_*: CHP_SLEEP_SLEEPING after Body.sleep() */
+  double sleep_speed_limit;              /* Body.sleepSpeedLimit (default 0.1) */
+  double sleep_time_limit;               /* Body.sleepTimeLimit (default 1) */
 } chp_body_desc;
 
 typedef struct chp_constraint_desc {     /* lib/constraints/ constructors */
@@ -175,7 +183,8 @@ typedef struct chp_body_state {
   float inv_inertia_world[9];            /* Matrix3 storage order */
   float aabb_lower[3], aabb_upper[3];
   int32_t aabb_needs_update;
-  int32_t reserved;
+  int32_t sleep_state;                   /* CHP_SLEEP_* */
+  double time_last_sleepy;               /* Body.timeLastSleepy */
 } chp_body_state;
 
 /* derived per-body constants, for setup parity (Body.updateMassProperties etc.) */
@@ -288,7 +297,12 @@ int chp_stage_times(chp_ctx* ctx, double* ms, const char** names, uint32_t cap, 
 typedef struct chp_step_stats {
   uint32_t pairs, contacts, rows, iterations, levels, status, steps;
   uint32_t sum_pairs, sum_rows, sum_contacts;   /* totals over every step taken */
+  uint32_t sleeping;                            /* bodies asleep after the last step */
+  uint32_t reserved;
+  double time;                                  /* World.time: the sum of the steps' dt */
 } chp_step_stats;
+/* World.time of a world (the sleep timer reads it); chp_state_upload does not change it */
+int chp_world_set_time(chp_ctx* ctx, uint32_t world_id, double time);
 int chp_world_stats(chp_ctx* ctx, uint32_t world_id, chp_step_stats* out);
 
 /* ---- slot-level entry points (one world, current device state) --------------------------- */

@@ -50,6 +50,8 @@ void orc_get_state(void* wp, chp_body_state* out) {
     for (int k = 0; k < 9; k++) s.inv_inertia_world[k] = b.invInertiaWorld.e[k];
     put3(s.aabb_lower, b.aabbLower); put3(s.aabb_upper, b.aabbUpper);
     s.aabb_needs_update = b.aabbNeedsUpdate ? 1 : 0;
+    s.sleep_state = b.sleepState;
+    s.time_last_sleepy = b.timeLastSleepy;
   }
 }
 
@@ -66,13 +68,20 @@ void orc_set_state(void* wp, const chp_body_state* in) {
     for (int k = 0; k < 9; k++) b.invInertiaWorld.e[k] = s.inv_inertia_world[k];
     b.aabbLower = get3(s.aabb_lower); b.aabbUpper = get3(s.aabb_upper);
     b.aabbNeedsUpdate = s.aabb_needs_update != 0;
+    b.sleepState = s.sleep_state;
+    b.timeLastSleepy = s.time_last_sleepy;
   }
 }
 
 void orc_step(void* wp, double dt, int n) {
   World& w = *(World*)wp;
-  for (int i = 0; i < n; i++) internal_step(w, (R)dt);
+  for (int i = 0; i < n; i++) {  /* World.step fixed stepping, world_class.dart:392-399 */
+    internal_step(w, (R)dt);
+    w.time += dt;
+  }
 }
+double orc_time(void* wp) { return ((World*)wp)->time; }
+void orc_set_time(void* wp, double t) { ((World*)wp)->time = t; }
 
 /* steps many worlds, one std::thread per world slice (the "one isolate per world" analogue) */
 void orc_step_many(void** worlds, int nworlds, double dt, int n, int threads) {

@@ -389,6 +389,7 @@ World* build_world(const chp_scene* sc) {
   w->tolerance = wd.tolerance;
   w->quatNormalizeSkip = wd.quat_normalize_skip;
   w->quatNormalizeFast = wd.quat_normalize_fast != 0;
+  w->allowSleep = wd.allow_sleep != 0;
   w->defaultCM.friction = wd.friction;
   w->defaultCM.restitution = wd.restitution;
   w->defaultCM.contactEquationStiffness = wd.contact_stiffness;
@@ -463,6 +464,10 @@ World* build_world(const chp_scene* sc) {
     b->angularDamping = d.angular_damping;
     b->fixedRotation = d.fixed_rotation != 0;
     b->isTrigger = d.is_trigger != 0;
+    b->allowSleep = d.allow_sleep != 0;
+    b->sleepState = d.sleep_state;
+    b->sleepSpeedLimit = (R)d.sleep_speed_limit;
+    b->sleepTimeLimit = d.sleep_time_limit;
     b->position = f3(d.position);
     b->velocity = f3(d.velocity);
     b->invMass = b->mass > (R)0 ? (R)1 / b->mass : (R)0;

@@ -1,9 +1,9 @@
 /*
  * ORACLE -- test infrastructure only (see oracle_math.h).
  * World.internalStep (world_class.dart:433-701): broadphase, narrowphase, equations, GSSolver,
- * damping and integration, restated from the reference in its own order.  Event bookkeeping
- * (collision matrix, overlap keepers, collide events) and sleep are left out: they do not
- * change the physics at allowSleep=false.
+ * damping, integration and the sleep update, restated from the reference in its own order.
+ * Event bookkeeping (collision matrix, overlap keepers, collide events) is left out: it does not
+ * change the physics.
  */
 #include <algorithm>
 #include <cstring>
@@ -21,10 +21,12 @@ void body_update_inertia_world(Body& b, bool force);
 /* broadphase                                                                                 */
 /* ------------------------------------------------------------------------------------------ */
 
-/* Broadphase.needBroadphaseCollision, broadphase.dart:44-63 (no sleeping bodies here) */
+/* Broadphase.needBroadphaseCollision, broadphase.dart:44-63 */
 static bool needBroadphaseCollision(const Body& a, const Body& b) {
   if ((a.group & b.mask) == 0 || (b.group & a.mask) == 0) return false;
-  if (a.type == CHP_BODY_STATIC && b.type == CHP_BODY_STATIC) return false;
+  if ((a.type == CHP_BODY_STATIC || a.sleepState == CHP_SLEEP_SLEEPING) &&
+      (b.type == CHP_BODY_STATIC || b.sleepState == CHP_SLEEP_SLEEPING))
+    return false;
   return true;
 }
 
@@ -957,7 +959,7 @@ static R computeB(Equation& e, R h) {
 
 /* Body.updateSolveMassProperties, rigid_body.dart:303-314 */
 static void updateSolveMass(Body& b) {
-  if (b.type == CHP_BODY_KINEMATIC) {
+  if (b.sleepState == CHP_SLEEP_SLEEPING || b.type == CHP_BODY_KINEMATIC) {
     b.invMassSolve = 0;
     setValues(b.invInertiaSolve, 0, 0, 0);
     b.invInertiaWorldSolve = M3();
@@ -1067,8 +1069,34 @@ static void constraint_update(Constraint& c) {
 }
 
 /* Body.integrate, rigid_body.dart:627-680 */
+/* Body.wakeUp / sleep / sleepTick, rigid_body.dart:263-300 */
+static void body_wake_up(Body& b) {
+  b.sleepState = CHP_SLEEP_AWAKE;
+  b.wakeUpAfterNarrowphase = false;
+}
+static void body_sleep(Body& b) {
+  b.sleepState = CHP_SLEEP_SLEEPING;
+  setValues(b.velocity, 0, 0, 0);
+  setValues(b.angularVelocity, 0, 0, 0);
+  b.wakeUpAfterNarrowphase = false;
+}
+static void body_sleep_tick(Body& b, double time) {
+  if (!b.allowSleep) return;
+  int sleepState = b.sleepState;
+  R speedSquared = length2(b.velocity) + length2(b.angularVelocity);
+  R speedLimitSquared = b.sleepSpeedLimit * b.sleepSpeedLimit;
+  if (sleepState == CHP_SLEEP_AWAKE && speedSquared < speedLimitSquared) {
+    b.sleepState = CHP_SLEEP_SLEEPY;
+    b.timeLastSleepy = time;
+  } else if (sleepState == CHP_SLEEP_SLEEPY && speedSquared > speedLimitSquared) {
+    body_wake_up(b);
+  } else if (sleepState == CHP_SLEEP_SLEEPY && time - b.timeLastSleepy > b.sleepTimeLimit) {
+    body_sleep(b);
+  }
+}
+
 static void body_integrate(Body& b, R dt, bool quatNormalize, bool fast) {
-  if (!(b.type == CHP_BODY_DYNAMIC || b.type == CHP_BODY_KINEMATIC)) return;
+  if (!(b.type == CHP_BODY_DYNAMIC || b.type == CHP_BODY_KINEMATIC) || b.sleepState == CHP_SLEEP_SLEEPING) return;
   R iMdt = b.invMass * dt;
   b.velocity.x = S(L(b.velocity.x) + L(b.force.x) * iMdt * L(b.linearFactor.x));
   b.velocity.y = S(L(b.velocity.y) + L(b.force.y) * iMdt * L(b.linearFactor.y));
@@ -1145,7 +1173,27 @@ void internal_step(World& w, R dt) {
       if (ma.restitution >= 0 && mb.restitution >= 0) c.restitution = ma.restitution * mb.restitution;
     }
     add(&c);
+    /* wake-up marks, :562-590 */
+    Body* bi = c.bi;
+    Body* bj = c.bj;
+    if (bi->allowSleep && bi->type == CHP_BODY_DYNAMIC && bi->sleepState == CHP_SLEEP_SLEEPING &&
+        bj->sleepState == CHP_SLEEP_AWAKE && bj->type != CHP_BODY_STATIC) {
+      R speedSquaredB = length2(bj->velocity) + length2(bj->angularVelocity);
+      R speedLimitSquaredB = bj->sleepSpeedLimit * bj->sleepSpeedLimit;
+      if (speedSquaredB >= speedLimitSquaredB * (R)2) bi->wakeUpAfterNarrowphase = true;
+    }
+    if (bj->allowSleep && bj->type == CHP_BODY_DYNAMIC && bj->sleepState == CHP_SLEEP_SLEEPING &&
+        bi->sleepState == CHP_SLEEP_AWAKE && bi->type != CHP_BODY_STATIC) {
+      R speedSquaredA = length2(bi->velocity) + length2(bi->angularVelocity);
+      R speedLimitSquaredA = bi->sleepSpeedLimit * bi->sleepSpeedLimit;
+      if (speedSquaredA >= speedLimitSquaredA * (R)2) bj->wakeUpAfterNarrowphase = true;
+    }
   }
+  for (Body* b : w.bodies)  /* wake up bodies, :617-624 */
+    if (b->wakeUpAfterNarrowphase) {
+      body_wake_up(*b);
+      b->wakeUpAfterNarrowphase = false;
+    }
   for (auto& c : w.constraints) {
     constraint_update(*c);
     for (Equation* e : c->equations) add(e);
@@ -1167,6 +1215,8 @@ void internal_step(World& w, R dt) {
   for (Body* b : w.bodies) { setValues(b->force, 0, 0, 0); setValues(b->torque, 0, 0, 0); }  /* clearForces */
   w.sapDirty = true;
   w.stepnumber += 1;
+  if (w.allowSleep)  /* sleeping update, :686-699 */
+    for (Body* b : w.bodies) body_sleep_tick(*b, w.time);
 }
 
 /* The Narrowphase slot for given pairs (narrow_phase.dart:634-721) with World's assembly of

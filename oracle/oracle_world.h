@@ -69,6 +69,13 @@ struct Body {
   int group = 1, mask = -1;
   bool collisionResponse = true, isTrigger = false, fixedRotation = false;
   int material = -1;
+  /* sleep, rigid_body.dart:162-176 (Dart doubles for the limits and the clock) */
+  bool allowSleep = true;
+  int sleepState = CHP_SLEEP_AWAKE;
+  R sleepSpeedLimit = (R)0.1;
+  double sleepTimeLimit = 1;
+  double timeLastSleepy = 0;
+  bool wakeUpAfterNarrowphase = false;
 };
 
 struct Jac { V3 spatial, rotational; };
@@ -139,6 +146,8 @@ struct World {
   int quatNormalizeSkip = 0;
   bool quatNormalizeFast = false;
   int stepnumber = 0;
+  bool allowSleep = false;   /* World.allowSleep */
+  double time = 0;           /* World.time */
   ContactMat defaultCM;
   std::vector<MaterialDesc> materials;
   std::map<std::pair<int, int>, ContactMat> cmTable;

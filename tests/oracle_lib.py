@@ -46,6 +46,8 @@ def lib(mode="f64"):
                                       P(C.c_int), P(C.c_int)]),
         "orc_solve": (C.c_int, [vp, P(A.Equation), C.c_int, C.c_double, P(C.c_double)]),
         "orc_sat": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_double)]),
+        "orc_time": (C.c_double, [vp]),
+        "orc_set_time": (None, [vp, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -123,6 +125,13 @@ class OracleWorld:
 
     def contacts(self):
         return self.L.orc_capture_contacts(self.ptr)
+
+    def time(self):
+        """World.time"""
+        return self.L.orc_time(self.ptr)
+
+    def set_time(self, t):
+        self.L.orc_set_time(self.ptr, t)
 
     def sap_axis(self):
         return self.L.orc_sap_axis(self.ptr)

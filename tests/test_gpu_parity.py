@@ -35,6 +35,10 @@ SCENES = {
     "feature_matrix_sap": (lambda: scenes.feature_matrix_world(8, "sap"), DT, 45),
     # 20-vertex prism caps: contact polygons beyond the thread path's clip buffers (wave path)
     "prism20": (lambda: scenes.prism_world(9), DT, 40),
+    # World.allowSleep: sleepy / sleeping transitions, wake-ups by contact, sleeping bodies out of
+    # the broadphase and weightless in the solve (the step count covers every transition)
+    "sleep": (lambda: scenes.sleep_world(21), DT, 170),
+    "sleep_sap": (lambda: scenes.sleep_world(22, "sap"), DT, 170),
 }
 
 _ctx = {}
@@ -263,3 +267,44 @@ def test_graph_replays_equal_eager_steps():
     finally:
         g.close()
         e.close()
+
+
+@pytest.mark.parametrize("mode", ["f32", "f64"])
+def test_sleep_states_clock_and_batching_match_oracle(mode):
+    """World.allowSleep: sleep state, Body.timeLastSleepy and World.time step by step against the
+    oracle, with the sleep world batched among worlds that never sleep (the wake-up pass runs for
+    part of the batch only); then a state upload that puts bodies to sleep and a new clock."""
+    rec = scenes.sleep_world(23).recipe()
+    c = ctx(mode)
+    others = [c.create_world(scenes.c3_world(60 + i).recipe()) for i in range(3)]
+    wid = c.create_world(rec)
+    try:
+        ow = O.OracleWorld(rec, mode)
+        seen = set()
+        for _ in range(12):
+            c.step(others[:1] + [wid] + others[1:], DT, 15)
+            ow.step(DT, 15)
+            g, o = state_to_numpy(*c.download(wid)), ow.state()
+            assert all(same_state(g, o).values()), same_state(g, o)
+            st = c.stats(wid)
+            assert st["time"] == ow.time()
+            assert st["sleeping"] == int(np.sum(o["sleep_state"] == A.SLEEP_SLEEPING))
+            seen.update(o["sleep_state"].tolist())
+        assert seen == {A.SLEEP_AWAKE, A.SLEEP_SLEEPY, A.SLEEP_SLEEPING}
+        # every awake body asleep from here, the clock set back: both sides from the same upload
+        arr, n = c.download(wid)
+        for i in range(n):
+            if arr[i].sleep_state == A.SLEEP_AWAKE:
+                arr[i].sleep_state = A.SLEEP_SLEEPING
+        c.upload(wid, arr, n)
+        ow.set_state_raw(arr)
+        c.set_time(wid, 0.25)
+        ow.set_time(0.25)
+        c.step([wid], DT, 30)
+        ow.step(DT, 30)
+        g, o = state_to_numpy(*c.download(wid)), ow.state()
+        assert all(same_state(g, o).values()), same_state(g, o)
+        assert c.stats(wid)["time"] == ow.time()
+    finally:
+        for w in others + [wid]:
+            c.destroy_world(w)

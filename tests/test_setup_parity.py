@@ -21,6 +21,7 @@ SCENES = {
     "c4_chain": lambda: scenes.c4_chain_world(7, length=25),
     "c5_terrain": lambda: scenes.c5_terrain_world(3, n=24),
     "feature_matrix": lambda: scenes.feature_matrix_world(7),
+    "sleep": lambda: scenes.sleep_world(21),
 }
 
 
