@@ -249,6 +249,29 @@ class Equation(C.Structure):
         ("restitution", C.c_double),
         ("angle_cos", C.c_double),
         ("target_velocity", C.c_double),
+        ("shape_i", C.c_int32),
+        ("shape_j", C.c_int32),
+    ]
+
+
+(EVENT_COLLIDE, EVENT_BEGIN_CONTACT, EVENT_END_CONTACT, EVENT_BEGIN_SHAPE_CONTACT,
+ EVENT_END_SHAPE_CONTACT) = range(5)
+EVENT_NAMES = ["collide", "beginContact", "endContact", "beginShapeContact", "endShapeContact"]
+
+
+class Event(C.Structure):
+    """chp_event: one contact event (collide, begin/endContact, begin/endShapeContact)"""
+    _fields_ = [
+        ("step", C.c_uint32),
+        ("kind", C.c_int32),
+        ("body_a", C.c_int32),
+        ("body_b", C.c_int32),
+        ("shape_a", C.c_int32),
+        ("shape_b", C.c_int32),
+        ("ni", F3),
+        ("ri", F3),
+        ("rj", F3),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -303,4 +326,5 @@ STRUCT_SIZES = {
     "chp_step_stats": C.sizeof(StepStats),
     "chp_equation": C.sizeof(Equation),
     "chp_request_status": C.sizeof(RequestStatus),
+    "chp_event": C.sizeof(Event),
 }

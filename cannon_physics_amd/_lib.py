@@ -56,12 +56,14 @@ def load():
         "chp_stage_times": (i32, [ctx_p, P(dbl), P(C.c_char_p), u32, P(u32)]),
         "chp_world_stats": (i32, [ctx_p, u32, P(A.StepStats)]),
         "chp_world_set_time": (i32, [ctx_p, u32, dbl]),
+        "chp_world_set_events": (i32, [ctx_p, u32, u32, u32]),
+        "chp_world_events": (i32, [ctx_p, u32, P(A.Event), u32, P(u32)]),
         "chp_broadphase": (i32, [ctx_p, u32, P(i32), P(i32), u32, P(u32)]),
         "chp_step_capture": (i32, [ctx_p, u32, dbl, P(i32), P(i32), u32, P(u32), P(A.Row), u32,
                                    P(u32), P(i32)]),
         "chp_narrowphase": (i32, [ctx_p, u32, P(i32), P(i32), u32, dbl, P(A.Equation), u32, P(u32), P(u32),
                                   P(u32)]),
-        "chp_solve": (i32, [ctx_p, u32, P(A.Equation), u32, dbl, P(i32), P(dbl)]),
+        "chp_solve": (i32, [ctx_p, u32, P(A.Equation), u32, dbl, i32, dbl, P(i32), P(dbl)]),
         "chp_serve_submit": (i32, [ctx_p, P(A.Scene), dbl, i32, P(u64)]),
         "chp_serve_tick": (i32, [ctx_p, i32, P(u32), P(u32)]),
         "chp_serve_status": (i32, [ctx_p, u64, P(A.RequestStatus)]),

@@ -17,6 +17,7 @@
 #include "../../include/cannon_hip.h"
 #include "chp_build.h"
 #include "chp_stages.h"
+#include "chp_events.h"
 
 using namespace chp;
 
@@ -41,7 +42,10 @@ __global__ __launch_bounds__(256) void k_bp(const DevWorld<R>* __restrict__ worl
   extern __shared__ double lds_d[];
   CHP_WORLD_PROLOGUE
   if ((W.broadphase == CHP_BROADPHASE_SAP) != SAP) return;
-  if (threadIdx.x == 0) W.counters[C_STAGING] = 0;
+  if (threadIdx.x == 0) {
+    W.counters[C_STAGING] = 0;
+    if (W.ev_mask) W.ev_ctl[EV_JT] = 0;
+  }
   if (W.nb >= BP_BIG) return;  // the grid-wide kernels below take large worlds
   st_prestep<R, SAP>(W);
   __syncthreads();
@@ -377,6 +381,17 @@ __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worl
   }
 }
 
+// contact-event bookkeeping after the narrowphase (worlds with events on); seed: only the key
+// sets, from the contacts the slab holds
+template <typename R>
+__global__ __launch_bounds__(1024) void k_events(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                 int seed) {
+  __shared__ int sh[1024];
+  CHP_WORLD_PROLOGUE
+  if (!W.ev_mask) return;
+  st_events(W, seed != 0, sh);
+}
+
 // the wake-up pass after the narrowphase (worlds that may hold sleeping bodies)
 template <typename R>
 __global__ __launch_bounds__(256) void k_wake(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list) {
@@ -541,6 +556,15 @@ struct WorldSlot {
   size_t bytes = 0;
   double last_dt = -1;
   bool may_sleep = false;  // World.allowSleep, or a body asleep: the step runs the wake-up pass
+  // contact events: one device allocation (keys, sort buffers, scan, justTest list, control
+  // words, events) while a kind mask is set
+  struct Events {
+    void* mem = nullptr;
+    size_t bytes = 0;
+    uint32_t mask = 0;
+    int cap = 0, kcap = 0, jcap = 0;
+  } ev;
+  std::vector<int> shape_body;   // Shape.body per scene shape
   std::vector<double> lin_damp, ang_damp;  // Body.linearDamping / angularDamping
   std::vector<chp_body_info> info;
   int pair_cap = 0, contact_cap = 0, row_cap = 0;
@@ -578,6 +602,7 @@ struct chp_ctx {
   int big_nspans = 0, big_ntiles = 0, big_nw = 0, big_rounds = 0;
   int n_sap = 0;   // SAP worlds in the current list
   int n_sleep = 0; // worlds in the current list that may hold sleeping bodies
+  int n_events = 0; // worlds in the current list with contact events on
   bool keep_rows = false;   // steps also write row records (the capture reads them)
   int* d_status = nullptr;
   const int** d_cptrs = nullptr;
@@ -600,7 +625,7 @@ struct chp_ctx {
     bool keep = false;
     double dt = 0;
     const void *dq = nullptr, *dqcls = nullptr, *dlist = nullptr, *dworlds = nullptr, *dbig = nullptr;
-    std::array<int, 6> big{};
+    std::array<int, 7> big{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } sg;
@@ -652,6 +677,9 @@ static void default_caps(int nb, int nhf, int& pairs, int& contacts, int& rows) 
   rows = (int)std::min<long long>(3 * c + 64, 1LL << 30);
 }
 
+template <typename R> static int events_alloc(chp_ctx* ctx, WorldSlot& S, uint32_t mask, int cap);
+static void events_free(chp_ctx* ctx, WorldSlot& S);
+
 template <typename R>
 static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pair_cap, int contact_cap, int row_cap) {
   Slab L;
@@ -673,6 +701,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   D.quat_fast = H.wd.quat_normalize_fast;
   D.has_filtered_constraints = H.has_filtered ? 1 : 0;
   D.allow_sleep = H.wd.allow_sleep ? 1 : 0;
+  D.nshapes = H.nshapes;
   for (int k = 0; k < 3; k++) D.gravity[k] = H.wd.gravity[k];
   {
     f3 g = H.wd.has_friction_gravity ? mk3(H.wd.friction_gravity[0], H.wd.friction_gravity[1], H.wd.friction_gravity[2])
@@ -706,7 +735,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
     std::vector<double> t0(1, H.time);
     ARR(wtime, t0);
   }
-  ARR(si_type, H.si_type); ARR(si_hull, H.si_hull); ARR(si_hf, H.si_hf); ARR(si_response, H.si_response);
+  ARR(si_type, H.si_type); ARR(si_shape, H.si_shape); ARR(si_hull, H.si_hull); ARR(si_hf, H.si_hf); ARR(si_response, H.si_response);
   ARR(si_group, H.si_group); ARR(si_mask, H.si_mask); ARR(si_material, H.si_material);
   ARR(si_off, H.si_off); ARR(si_ori, H.si_ori); ARR(si_half, H.si_half); ARR(si_radius, H.si_radius); ARR(si_bsr, H.si_bsr);
   ARR(h_vstart, H.H.vstart); ARR(h_vcount, H.H.vcount); ARR(h_fstart, H.H.fstart); ARR(h_fcount, H.H.fcount);
@@ -764,7 +793,12 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   S.last_dt = -1;
   S.may_sleep = H.wd.allow_sleep != 0;
   for (int v : H.sleep_state) S.may_sleep = S.may_sleep || v == CHP_SLEEP_SLEEPING;
+  S.shape_body = H.shape_body;
   S.devworld.assign((char*)&D, (char*)&D + sizeof(D));
+  if (S.ev.mask) {   // a rebuilt slab keeps the event bookkeeping (sized for the new capacities)
+    int r = events_alloc<R>(ctx, S, S.ev.mask, S.ev.cap);
+    if (r != CHP_OK) return r;
+  }
   ctx->worlds_dirty = true;
   ctx->list_valid = false;
   return CHP_OK;
@@ -825,6 +859,7 @@ static void release_world(chp_ctx* ctx, uint32_t id) {
   WorldSlot& S = ctx->worlds[id];
   ctx->arena.release(S.mem, S.bytes);
   ctx->in_use -= S.bytes;
+  events_free(ctx, S);
   S = WorldSlot();
   ctx->worlds_dirty = true;
   ctx->list_valid = false;
@@ -833,6 +868,101 @@ static void release_world(chp_ctx* ctx, uint32_t id) {
 static bool valid_world(chp_ctx* ctx, uint32_t id) { return ctx && id < ctx->worlds.size() && ctx->worlds[id].live; }
 
 template <typename R> static DevWorld<R>& dev_of(WorldSlot& S) { return *(DevWorld<R>*)S.devworld.data(); }
+
+// ---- contact events: the per-world device allocation -----------------------------------------
+// layout: keys 4 * kcap u64 | sort buffers 2 * kcap u64 | scan kcap + 1 ints | justTest list
+// jcap int4 | control words | events cap
+struct EvLayout {
+  size_t keys, tmp, scan, jt, ctl, buf, bytes;
+  EvLayout(int kcap, int jcap, int cap) {
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    keys = 0;
+    tmp = up(keys + 4 * (size_t)kcap * 8);
+    scan = up(tmp + 2 * (size_t)kcap * 8);
+    jt = up(scan + ((size_t)kcap + 1) * 4);
+    ctl = up(jt + (size_t)jcap * 16);
+    buf = up(ctl + EV_WORDS * 4);
+    bytes = up(buf + (size_t)cap * sizeof(chp_event));
+  }
+};
+static void events_free(chp_ctx* ctx, WorldSlot& S) {
+  if (S.ev.mem) {
+    (void)hipFree(S.ev.mem);
+    ctx->in_use -= S.ev.bytes;
+  }
+  S.ev = WorldSlot::Events();
+}
+// the world's DevWorld image pointed at its event buffers (or cleared)
+template <typename R> static void events_attach(chp_ctx* ctx, WorldSlot& S) {
+  DevWorld<R>& D = dev_of<R>(S);
+  if (!S.ev.mem) {
+    auto clear = [](auto& field) { *(void**)(void*)&field = nullptr; };
+    clear(D.ev_keys); clear(D.ev_tmp); clear(D.ev_scan); clear(D.ev_jt); clear(D.ev_ctl); clear(D.ev_buf);
+    D.ev_mask = D.ev_cap = D.ev_kcap = 0;
+  } else {
+    EvLayout L(S.ev.kcap, S.ev.jcap, S.ev.cap);
+    char* m = (char*)S.ev.mem;
+    auto set = [](auto& field, void* p) { *(void**)(void*)&field = p; };   // device pointers
+    set(D.ev_keys, m + L.keys); set(D.ev_tmp, m + L.tmp); set(D.ev_scan, m + L.scan);
+    set(D.ev_jt, m + L.jt); set(D.ev_ctl, m + L.ctl); set(D.ev_buf, m + L.buf);
+    D.ev_mask = (int)S.ev.mask; D.ev_cap = S.ev.cap; D.ev_kcap = S.ev.kcap;
+  }
+  ctx->worlds_dirty = true;
+  ctx->list_valid = false;
+}
+// buffers for the world's current capacities and the given event capacity; a live allocation
+// that is too small moves into the new one (key sets, control words, pending events)
+template <typename R> static int events_alloc(chp_ctx* ctx, WorldSlot& S, uint32_t mask, int cap) {
+  const int kcap = S.contact_cap + S.pair_cap, jcap = std::max(S.pair_cap, 1);
+  if (cap <= 0) cap = S.contact_cap + 4 * kcap;   // one step at full capacities
+  if (S.ev.mem && S.ev.kcap >= kcap && S.ev.jcap >= jcap && S.ev.cap == cap) {
+    S.ev.mask = mask;
+    events_attach<R>(ctx, S);
+    return CHP_OK;
+  }
+  EvLayout L(kcap, jcap, cap);
+  if (ctx->in_use + L.bytes > ctx->budget)
+    return fail(ctx, CHP_E_OOM, "event buffers need " + std::to_string(L.bytes) + " bytes beyond the context budget");
+  void* mem = nullptr;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  hipError_t e = hipMalloc(&mem, L.bytes);
+  if (e != hipSuccess) return fail(ctx, CHP_E_OOM, std::string("event buffers: ") + hipGetErrorString(e));
+  e = hipMemset(mem, 0, L.bytes);
+  if (e == hipSuccess && S.ev.mem) {   // carry the live bookkeeping over
+    EvLayout O(S.ev.kcap, S.ev.jcap, S.ev.cap);
+    char *dst = (char*)mem, *src = (char*)S.ev.mem;
+    const int nk = std::min(S.ev.kcap, kcap);   // key sets beyond a smaller capacity are cut
+    for (int k = 0; k < 4 && e == hipSuccess; k++)
+      e = hipMemcpy(dst + L.keys + (size_t)k * kcap * 8, src + O.keys + (size_t)k * S.ev.kcap * 8, (size_t)nk * 8,
+                    hipMemcpyDeviceToDevice);
+    int ctl[EV_WORDS];
+    if (e == hipSuccess) e = hipMemcpy(ctl, src + O.ctl, sizeof ctl, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+      for (int k = EV_NBODY; k < EV_NBODY + 4; k++) ctl[k] = std::min(ctl[k], nk);
+      const int keep = std::min(ctl[EV_N], cap);   // pending events beyond the new capacity are dropped
+      if (keep < ctl[EV_N]) ctl[EV_NEED] = std::max(ctl[EV_NEED], ctl[EV_N]);
+      ctl[EV_N] = keep;
+      if (keep > 0) e = hipMemcpy(dst + L.buf, src + O.buf, (size_t)keep * sizeof(chp_event), hipMemcpyDeviceToDevice);
+      if (e == hipSuccess) e = hipMemcpy(dst + L.ctl, ctl, sizeof ctl, hipMemcpyHostToDevice);
+    }
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(mem);
+    return fail(ctx, CHP_E_HIP, std::string("event buffers: ") + hipGetErrorString(e));
+  }
+  const bool had = S.ev.mem != nullptr;
+  events_free(ctx, S);
+  S.ev.mem = mem;
+  S.ev.bytes = L.bytes;
+  S.ev.mask = mask;
+  S.ev.cap = cap;
+  S.ev.kcap = kcap;
+  S.ev.jcap = jcap;
+  ctx->in_use += L.bytes;
+  (void)had;
+  events_attach<R>(ctx, S);
+  return CHP_OK;
+}
 
 // the damping factors of a world for dt (world_class.dart:652-656: pow(1 - damping, dt),
 // evaluated once per (body, dt) on the host); no device allocation
@@ -968,10 +1098,12 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   int max_tiles = 1;
   ctx->n_sap = 0;
   ctx->n_sleep = 0;
+  ctx->n_events = 0;
   for (size_t li = 0; li < n; li++) {
     const WorldSlot& S = ctx->worlds[list[li]];
     if (S.broadphase == CHP_BROADPHASE_SAP) ctx->n_sap++;
     if (S.may_sleep) ctx->n_sleep++;
+    if (S.ev.mask) ctx->n_events++;
     if (S.nb < BP_BIG) continue;
     bigw.push_back((int)li);
     for (int s0 = 0; s0 < S.nb; s0 += BP_SPAN) { spans.push_back((int)li); spans.push_back(s0); }
@@ -1013,7 +1145,7 @@ struct StepView {
   const void* worlds = nullptr;
   const int* list = nullptr;
   const int* big = nullptr;
-  int nspans = 0, ntiles = 0, nw = 0, rounds = 0, n_sap = 0, n_sleep = 0;
+  int nspans = 0, ntiles = 0, nw = 0, rounds = 0, n_sap = 0, n_sleep = 0, n_events = 0;
   int2* q = nullptr;
   size_t qcap = 0;
   int* qcls = nullptr;
@@ -1029,6 +1161,7 @@ static StepView ctx_view(const chp_ctx* ctx) {
   v.nspans = ctx->big_nspans; v.ntiles = ctx->big_ntiles; v.nw = ctx->big_nw; v.rounds = ctx->big_rounds;
   v.n_sap = ctx->n_sap;
   v.n_sleep = ctx->n_sleep;
+  v.n_events = ctx->n_events;
   v.q = ctx->d_q;
   v.qcap = ctx->d_q_cap;
   v.qcls = ctx->d_qcls;
@@ -1101,6 +1234,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
     // compound-body pairs are rare: a small grid, strided over however many there are
     k_narrow_wave<R, NP_WAVE><<<32, 256, 0, st>>>(dw, V.q, bounds, dt);
     if (V.n_sleep) k_wake<R><<<n, 256, 0, st>>>(dw, d_list);
+    if (V.n_events) k_events<R><<<n, 1024, 0, st>>>(dw, d_list, 0);
   }
   if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
   // rows: offsets, then the rows' body pairs; the level schedule; then the rows, written as GS
@@ -1165,7 +1299,8 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
   *out = nullptr;
   if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block && g.keep == ctx->keep_rows &&
         g.bg == ctx->solve_block && g.dq == ctx->d_q && g.dqcls == ctx->d_qcls && g.dlist == ctx->d_list &&
-        g.dbig == ctx->d_big && g.big == std::array<int, 6>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep} &&
+        g.dbig == ctx->d_big && g.big == std::array<int, 7>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep,
+                                       ctx->n_events} &&
         g.dworlds == ctx->d_worlds)) {
     if (!may_capture) return CHP_OK;
     drop_graph(ctx);
@@ -1184,7 +1319,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
     g.valid = true; g.list = list; g.nbig = nbig; g.max_nb = max_nb; g.dt = dt;
     g.bl = ctx->levels_block; g.bg = ctx->solve_block; g.keep = ctx->keep_rows;
     g.dq = ctx->d_q; g.dqcls = ctx->d_qcls; g.dlist = ctx->d_list; g.dworlds = ctx->d_worlds;
-    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep};
+    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep, ctx->n_events};
     g.graph = graph;
     g.exec = exec;
   }
@@ -1367,6 +1502,7 @@ template <typename R> static int step_noalloc_t(chp_ctx* ctx, uint32_t id, doubl
   V.nspans = nspans; V.ntiles = ntiles; V.nw = big ? 1 : 0; V.rounds = rounds;
   V.n_sap = S.broadphase == CHP_BROADPHASE_SAP ? 1 : 0;
   V.n_sleep = S.may_sleep ? 1 : 0;
+  V.n_events = S.ev.mask ? 1 : 0;
   V.qcls = (int*)(base + o_qcls);
   V.q = (int2*)(base + o_q);
   V.qcap = (size_t)S.pair_cap;
@@ -1625,9 +1761,26 @@ static int narrowphase_t(chp_ctx* ctx, uint32_t id, const int32_t* pi, const int
 }
 
 template <typename R>
-static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t n, double dt, int32_t* iters,
-                   double* mult) {
+static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t n, double dt, int32_t max_iters,
+                   double tol, int32_t* iters, double* mult) {
   WorldSlot& S = ctx->worlds[id];
+  // GSSolver.iterations / .tolerance of the calling solver, for this call only
+  struct Restore {
+    chp_ctx* ctx;
+    DevWorld<R>& D;
+    int it;
+    R tol2;
+    ~Restore() {
+      if (D.iterations != it || D.tol2 != tol2) {
+        D.iterations = it;
+        D.tol2 = tol2;
+        ctx->worlds_dirty = true;
+      }
+    }
+  } restore{ctx, dev_of<R>(S), dev_of<R>(S).iterations, dev_of<R>(S).tol2};
+  if (max_iters >= 0) restore.D.iterations = max_iters;
+  if (tol >= 0) restore.D.tol2 = (R)tol * (R)tol;
+  if (restore.D.iterations != restore.it || restore.D.tol2 != restore.tol2) ctx->worlds_dirty = true;
   if (n > (uint32_t)S.row_cap)
     return fail(ctx, CHP_E_CAPACITY, "more equations than the world's row slab (" + std::to_string(S.row_cap) +
                                          "); raise it with chp_world_set_capacity");
@@ -1816,6 +1969,29 @@ template <typename R> static int inspect_t(const chp_scene* scene, chp_body_info
   return CHP_OK;
 }
 
+template <typename R> static int set_events_t(chp_ctx* ctx, uint32_t id, uint32_t mask, uint32_t cap) {
+  WorldSlot& S = ctx->worlds[id];
+  if (mask == 0) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    events_free(ctx, S);
+    events_attach<R>(ctx, S);
+    return CHP_OK;
+  }
+  const bool fresh = S.ev.mem == nullptr;
+  int r = events_alloc<R>(ctx, S, mask, (int)std::min<uint32_t>(cap, 1u << 30));
+  if (r != CHP_OK) return r;
+  if (fresh) {   // the keepers' first sets: the contact list of the last step taken
+    r = prepare<R>(ctx, &id, 1, ctx->worlds[id].last_dt > 0 ? ctx->worlds[id].last_dt : 1.0 / 60);
+    if (r != CHP_OK) return r;
+    r = upload_list(ctx, {(int)id});
+    if (r != CHP_OK) return r;
+    k_events<R><<<1, 1024, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, ctx->d_list, 1);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return CHP_OK;
+}
+
 extern "C" {
 
 int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out) {
@@ -1848,6 +2024,7 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
+  for (WorldSlot& S : ctx->worlds) events_free(ctx, S);
   ctx->arena.destroy();
   if (ctx->d_worlds) (void)hipFree(ctx->d_worlds);
   if (ctx->d_big) (void)hipFree(ctx->d_big);
@@ -1887,6 +2064,7 @@ int chp_world_destroy(chp_ctx* ctx, uint32_t id) {
   WorldSlot& S = ctx->worlds[id];
   ctx->arena.release(S.mem, S.bytes);
   ctx->in_use -= S.bytes;
+  events_free(ctx, S);
   S = WorldSlot();
   ctx->worlds_dirty = true;
   ctx->list_valid = false;
@@ -2008,6 +2186,56 @@ int chp_world_stats(chp_ctx* ctx, uint32_t id, chp_step_stats* out) {
   return ctx->precision == CHP_PRECISION_F64 ? sleep_stats_t<double>(ctx, id, out) : sleep_stats_t<float>(ctx, id, out);
 }
 
+int chp_world_set_events(chp_ctx* ctx, uint32_t id, uint32_t mask, uint32_t capacity) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (mask >> (CHP_EVENT_END_SHAPE_CONTACT + 1)) return fail(ctx, CHP_E_INVALID, "unknown event kind in the mask");
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? set_events_t<double>(ctx, id, mask, capacity)
+                                             : set_events_t<float>(ctx, id, mask, capacity);
+}
+
+int chp_world_events(chp_ctx* ctx, uint32_t id, chp_event* out, uint32_t cap, uint32_t* n) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id) || !n) return fail(ctx, CHP_E_INVALID, "bad argument");
+  WorldSlot& S = ctx->worlds[id];
+  *n = 0;
+  if (!S.ev.mem) return CHP_OK;
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  EvLayout L(S.ev.kcap, S.ev.jcap, S.ev.cap);
+  char* m = (char*)S.ev.mem;
+  int ctl[EV_WORDS];
+  HIPCHK(ctx, hipMemcpy(ctl, m + L.ctl, sizeof ctl, hipMemcpyDeviceToHost));
+  const int stored = ctl[EV_N], raised = ctl[EV_NEED];
+  if (!out) {   // a query: the number stored
+    *n = (uint32_t)stored;
+    return CHP_OK;
+  }
+  if (raised > stored) {   // the buffer overflowed: the events are dropped
+    ctl[EV_N] = ctl[EV_NEED] = 0;
+    HIPCHK(ctx, hipMemcpy(m + L.ctl, ctl, sizeof ctl, hipMemcpyHostToDevice));
+    *n = (uint32_t)raised;
+    return fail(ctx, CHP_E_CAPACITY, "the event buffer overflowed: " + std::to_string(raised) +
+                                         " events were raised since the last drain; they are dropped");
+  }
+  *n = (uint32_t)stored;
+  if ((uint32_t)stored > cap) return fail(ctx, CHP_E_CAPACITY, "event output buffer too small: need " + std::to_string(stored));
+  if (stored) HIPCHK(ctx, hipMemcpy(out, m + L.buf, (size_t)stored * sizeof(chp_event), hipMemcpyDeviceToHost));
+  for (int k = 0; k < stored; k++) {   // shape events name the shapes' bodies (Shape.body)
+    chp_event& e = out[k];
+    if (e.kind == CHP_EVENT_BEGIN_SHAPE_CONTACT || e.kind == CHP_EVENT_END_SHAPE_CONTACT) {
+      e.body_a = e.shape_a >= 0 && e.shape_a < (int)S.shape_body.size() ? S.shape_body[e.shape_a] : -1;
+      e.body_b = e.shape_b >= 0 && e.shape_b < (int)S.shape_body.size() ? S.shape_body[e.shape_b] : -1;
+    }
+  }
+  ctl[EV_N] = ctl[EV_NEED] = 0;
+  HIPCHK(ctx, hipMemcpy(m + L.ctl, ctl, sizeof ctl, hipMemcpyHostToDevice));
+  return CHP_OK;
+}
+
 int chp_world_set_time(chp_ctx* ctx, uint32_t id, double time) {
   if (!ctx) return CHP_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->mu);
@@ -2053,15 +2281,17 @@ int chp_narrowphase(chp_ctx* ctx, uint32_t id, const int32_t* pi, const int32_t*
                                              : narrowphase_t<float>(ctx, id, pi, pj, npairs, dt, out, cap, neq, nf, nc);
 }
 
-int chp_solve(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t n, double dt, int32_t* iterations,
-              double* multipliers) {
+int chp_solve(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t n, double dt, int32_t max_iterations,
+              double tolerance, int32_t* iterations, double* multipliers) {
   if (!ctx) return CHP_E_INVALID;
   std::lock_guard<std::mutex> g(ctx->mu);
   if (!valid_world(ctx, id) || (n && !eqs) || !iterations) return fail(ctx, CHP_E_INVALID, "bad argument");
   if (!(dt > 0)) return fail(ctx, CHP_E_INVALID, "dt must be > 0");
   (void)hipSetDevice(ctx->device);
-  return ctx->precision == CHP_PRECISION_F64 ? solve_t<double>(ctx, id, eqs, n, dt, iterations, multipliers)
-                                             : solve_t<float>(ctx, id, eqs, n, dt, iterations, multipliers);
+  if (!std::isfinite(tolerance)) return fail(ctx, CHP_E_INVALID, "tolerance must be finite");
+  return ctx->precision == CHP_PRECISION_F64
+             ? solve_t<double>(ctx, id, eqs, n, dt, max_iterations, tolerance, iterations, multipliers)
+             : solve_t<float>(ctx, id, eqs, n, dt, max_iterations, tolerance, iterations, multipliers);
 }
 
 int chp_serve_submit(chp_ctx* ctx, const chp_scene* scene, double dt, int32_t nsteps, uint64_t* ticket) {
@@ -2219,6 +2449,7 @@ uint32_t chp_abi_struct_size(const char* name) {
   SZ(chp_world_desc); SZ(chp_material_desc); SZ(chp_contact_material_desc); SZ(chp_shape_desc); SZ(chp_hull_desc);
   SZ(chp_heightfield_desc); SZ(chp_body_shape); SZ(chp_body_desc); SZ(chp_constraint_desc); SZ(chp_scene);
   SZ(chp_body_state); SZ(chp_body_info); SZ(chp_row); SZ(chp_step_stats); SZ(chp_equation); SZ(chp_request_status);
+  SZ(chp_event);
 #undef SZ
   return 0;
 }

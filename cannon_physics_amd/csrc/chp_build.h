@@ -98,6 +98,9 @@ template <typename R> struct HostWorld {
   std::vector<R> sleep_speed;
   // shape instances
   std::vector<int> si_type, si_hull, si_hf, si_response, si_group, si_mask, si_material;
+  std::vector<int> si_shape;   // the scene shape of each instance
+  std::vector<int> shape_body; // Shape.body: the last body that added each scene shape (-1: none)
+  int nshapes = 0;
   std::vector<float> si_off, si_ori, si_half;
   std::vector<R> si_radius, si_bsr;
   HullBuf H;
@@ -367,6 +370,8 @@ template <typename R> struct Builder {
   std::vector<BodyTmp> bodies;
 
   void build_bodies() {
+    W.shape_body.assign(S.num_shapes, -1);
+    W.nshapes = (int)S.num_shapes;
     W.nb = (int)S.num_bodies;
     for (uint32_t i = 0; i < S.num_bodies; i++) {  // Body constructor, rigid_body.dart:27-86
       const chp_body_desc& d = S.bodies[i];
@@ -426,6 +431,8 @@ template <typename R> struct Builder {
         const chp_shape_desc& sd = S.shapes[bs.shape];
         if (sd.material >= (int)S.num_materials) fail("shape material out of range");
         W.si_type.push_back(sd.type);
+        W.si_shape.push_back((int)bs.shape);
+        W.shape_body[bs.shape] = (int)i;   // Body.addShape sets shape.body
         W.si_hull.push_back(shape_hull[bs.shape]);
         W.si_hf.push_back(sd.type == CHP_SHAPE_HEIGHTFIELD ? hf_index[sd.index] : -1);
         W.si_response.push_back(sd.collision_response ? 1 : 0);

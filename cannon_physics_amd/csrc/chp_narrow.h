@@ -38,7 +38,7 @@ template <typename R> CHP_HD HullView<R> hull_view(const DevWorld<R>& W, int h) 
   return H;
 }
 
-struct SRef { int resp; int mat; };   // Shape.collisionResponse / Shape.material
+struct SRef { int resp; int mat; int id; };   // Shape.collisionResponse / .material, instance
 
 // where resolvers put contacts.  WAVE: every lane of a wave computes the same contacts and
 // lane 0 stores them.  Two kinds:
@@ -125,6 +125,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   CHP_HD void emit(int bi, int bj, SRef si, SRef sj, SRef rsi, SRef rsj, f3 ri, f3 rj, f3 ni) {
     ContactRec<R> c;
     c.bi = bi; c.bj = bj; c.cm = cm;
+    c.si = rsi.id; c.sj = rsj.id;   // createContactEquation: c.si = overrideShapeA ?? si
     c.ri = ri; c.rj = rj; c.ni = ni;
     const R* cmp = W.cm_params + 6 * cm;
     bool enabled = (W.flags[bi] & BF_RESPONSE) && (W.flags[bj] & BF_RESPONSE) && si.resp && sj.resp;
@@ -725,7 +726,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     R mn, mx;
     if (!hfWindow(hf, local, Rr, x0, x1, y0, y1, mn, mx)) return;
     if (ld<R>(local.z) - Rr > mx || ld<R>(local.z) + Rr < mn) return;
-    SRef pref{1, -1};
+    SRef pref{1, -1, sj.id};   // the pillar hull (the contacts carry the override shapes)
     for (int i = x0; i < x1; i++)
       for (int j = y0; j < y1; j++) {
         int before = sink->n;
@@ -739,14 +740,15 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
       }
   }
   // heightfieldConvex (:2044-2178): convexConvex against each pillar with faceListA = [0]
-  CHP_HD void heightfieldConvex(const HullView<R>& A, SRef siRef, int hf, f3 xi, f3 xj, f4 qi, f4 qj, int bi, int bj) {
+  CHP_HD void heightfieldConvex(const HullView<R>& A, SRef siRef, int hf, int hfId, f3 xi, f3 xj, f4 qi, f4 qj, int bi,
+                                int bj) {
     f3 local = point_to_local<R>(xj, qj, xi);
     int x0, x1, y0, y1;
     R mn, mx;
     R radius = A.bsr;
     if (!hfWindow(hf, local, radius, x0, x1, y0, y1, mn, mx)) return;
     if (ld<R>(local.z) - radius > mx || ld<R>(local.z) + radius < mn) return;
-    SRef pref{1, -1};
+    SRef pref{1, -1, hfId};   // the pillar hull; its contacts name the heightfield
     for (int i = x0; i < x1; i++)
       for (int j = y0; j < y1; j++)
         for (int up = 0; up < 2; up++) {
@@ -762,7 +764,12 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     int ti = W.type[bi], tj = W.type[bj];
     bool justTest = (ti == CHP_BODY_KINEMATIC && tj == CHP_BODY_STATIC) || (ti == CHP_BODY_STATIC && tj == CHP_BODY_KINEMATIC) ||
                     (ti == CHP_BODY_KINEMATIC && tj == CHP_BODY_KINEMATIC);
-    if (justTest) return;  // justTest resolvers create no contacts
+    // justTest resolvers create no contacts.  With events on, the wave path (which the queue
+    // gives those pairs) records their overlaps; the thread path never sees them then.
+    if (justTest) {
+      if constexpr (!WAVE) return;
+      else if (!W.ev_mask) return;
+    }
     int bodyCM = -1;
     int mi = W.material[bi], mj = W.material[bj];
     if (mi >= 0 && mj >= 0) bodyCM = W.cm_table[mi * W.nmat + mj];
@@ -780,7 +787,30 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
         int shapeCM = -1;
         if (W.si_material[sa] >= 0 && W.si_material[sb] >= 0) shapeCM = W.cm_table[W.si_material[sa] * W.nmat + W.si_material[sb]];
         cm = shapeCM >= 0 ? shapeCM : (bodyCM >= 0 ? bodyCM : 0);
-        SRef rsi{W.si_response[sa], W.si_material[sa]}, rsj{W.si_response[sb], W.si_material[sb]};
+        SRef rsi{W.si_response[sa], W.si_material[sa], sa}, rsj{W.si_response[sb], W.si_material[sb], sb};
+        if constexpr (WAVE) if (justTest) {
+          // the resolver's justTest answer: sphereSphere tests the distance
+          // (narrow_phase.dart:737-739), every other resolver whether it would create a contact
+          bool hit;
+          if (W.si_type[sa] == CHP_SHAPE_SPHERE && W.si_type[sb] == CHP_SHAPE_SPHERE) {
+            const R dx = ld<R>(xi.x) - ld<R>(xj.x), dy = ld<R>(xi.y) - ld<R>(xj.y), dz = ld<R>(xi.z) - ld<R>(xj.z);
+            const R rr = W.si_radius[sa] + W.si_radius[sb];
+            hit = dx * dx + dy * dy + dz * dz < rr * rr;
+          } else {
+            Sink<R, WAVE> probe{nullptr, 0, 0, 0};   // counts, stores nothing
+            Sink<R, WAVE>* keep = sink;
+            sink = &probe;
+            if (W.si_type[sa] < W.si_type[sb]) dispatch(sa, sb, xi, xj, qi, qj, bi, bj, rsi, rsj);
+            else dispatch(sb, sa, xj, xi, qj, qi, bj, bi, rsi, rsj);
+            sink = keep;
+            hit = probe.n > 0;
+          }
+          if (hit && (threadIdx.x & 63) == 0) {   // Narrowphase.getContacts :712-716
+            const int k = atomicAdd(&W.ev_ctl[EV_JT], 1);
+            if (k < W.pair_cap) W.ev_jt[k] = make_int4(bi, bj, sa, sb);
+          }
+          continue;
+        }
         if (W.si_type[sa] < W.si_type[sb]) dispatch(sa, sb, xi, xj, qi, qj, bi, bj, rsi, rsj);
         else dispatch(sb, sa, xj, xi, qj, qi, bj, bi, rsi, rsj);
       }
@@ -789,7 +819,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
   // getCollisionType + operator[] (narrow_phase.dart:116-238, 476-489); type(A) <= type(B)
   CHP_HD void dispatch(int A, int B, f3 xa, f3 xb, f4 qa, f4 qb, int ba, int bb, SRef rsi, SRef rsj) {
     int ta = W.si_type[A], tb = W.si_type[B];
-    SRef ra{W.si_response[A], W.si_material[A]}, rb{W.si_response[B], W.si_material[B]};
+    SRef ra{W.si_response[A], W.si_material[A], A}, rb{W.si_response[B], W.si_material[B], B};
     const bool convA = ta == CHP_SHAPE_BOX || ta == CHP_SHAPE_CONVEX;
     if ((KINDS & NK_SS) && ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_SPHERE)
       sphereSphere(W.si_radius[A], W.si_radius[B], xa, xb, ba, bb, ra, rb, rsi, rsj);
@@ -806,7 +836,7 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     else if ((KINDS & NK_CC) && convA && (tb == CHP_SHAPE_BOX || tb == CHP_SHAPE_CONVEX))
       convexConvex(hull_view(W, W.si_hull[A]), ra, hull_view(W, W.si_hull[B]), rb, xa, xb, qa, qb, ba, bb, rsi, rsj, false);
     else if ((KINDS & NK_HC) && convA && tb == CHP_SHAPE_HEIGHTFIELD)
-      heightfieldConvex(hull_view(W, W.si_hull[A]), ra, W.si_hf[B], xa, xb, qa, qb, ba, bb);
+      heightfieldConvex(hull_view(W, W.si_hull[A]), ra, W.si_hf[B], B, xa, xb, qa, qb, ba, bb);
     // other combinations have no resolver in the covered set (narrow_phase.dart:699-700 skips)
   }
 };

@@ -627,6 +627,11 @@ template <typename R> __device__ void st_narrow(const DevWorld<R>& W, R dt, f3* 
 // compiled with the convex resolvers only; NP_WAVE, the rest (compound bodies).
 enum { NP_SS = 0, NP_SP, NP_SB, NP_SC, NP_SH, NP_PC, NP_CC, NP_OTHER, NP_WHC, NP_WAVE, NP_NCLS };
 template <typename R> __device__ __forceinline__ int pair_class(const DevWorld<R>& W, int bi, int bj) {
+  if (W.ev_mask) {   // justTest pairs (narrow_phase.dart:665-668): their overlap probe is on the wave path
+    const int ti = W.type[bi], tj = W.type[bj];
+    if ((ti == CHP_BODY_KINEMATIC && tj != CHP_BODY_DYNAMIC) || (tj == CHP_BODY_KINEMATIC && ti != CHP_BODY_DYNAMIC))
+      return NP_WAVE;
+  }
   if (W.shape_count[bi] != 1 || W.shape_count[bj] != 1) return wave_pair(W, bi, bj) ? NP_WAVE : NP_OTHER;
   int ta = W.si_type[W.shape_first[bi]], tb = W.si_type[W.shape_first[bj]];
   int lo = min(ta, tb), hi = max(ta, tb);
@@ -1055,6 +1060,7 @@ template <typename R> __device__ void st_equations(const DevWorld<R>& W, R h, ch
       chp_equation e;
       memset(&e, 0, sizeof e);
       e.body_i = c.bi; e.body_j = c.bj;
+      e.shape_i = e.shape_j = -1;   // FrictionEquation has no shapes
       e.ri[0] = c.ri.x; e.ri[1] = c.ri.y; e.ri[2] = c.ri.z;
       e.rj[0] = c.rj.x; e.rj[1] = c.rj.y; e.rj[2] = c.rj.z;
       if (c.flags & 2) {
@@ -1073,6 +1079,7 @@ template <typename R> __device__ void st_equations(const DevWorld<R>& W, R h, ch
       R ca, cb, ce;
       spook<R>(cm[2], cm[3], h, ca, cb, ce);
       e.kind = CHP_ROW_CONTACT;
+      e.shape_i = W.si_shape[c.si]; e.shape_j = W.si_shape[c.sj];
       e.n[0] = c.ni.x; e.n[1] = c.ni.y; e.n[2] = c.ni.z;
       e.min_force = 0.0; e.max_force = 1e6;
       e.a = (double)ca; e.b = (double)cb; e.eps = (double)ce;

@@ -10,6 +10,7 @@
 
 #include <vector>
 
+#include "../../include/cannon_hip.h"
 #include "chp_math.h"
 
 namespace chp {
@@ -33,6 +34,10 @@ enum : int {
   BF_ALLOW_SLEEP = 16,   // Body.allowSleep
   BF_WAKE = 32,          // Body.wakeUpAfterNarrowphase
 };
+
+// control words of a world's event bookkeeping: key-set parity, the sizes of the four key sets
+// (body / shape, parity 0 / 1), justTest overlaps of the step, events stored and raised
+enum : int { EV_PARITY = 0, EV_NBODY = 1, EV_NSHAPE = 3, EV_JT = 5, EV_N = 6, EV_NEED = 7, EV_WORDS = 8 };
 
 enum : int { WS_OK = 0, WS_PAIRS = 1, WS_CONTACTS = 2, WS_ROWS = 4, WS_NONFINITE = 8, WS_STAGING = 16 };
 // WS_STAGING: a contact polygon outgrew its clipping buffer (faces are limited at world creation
@@ -58,6 +63,7 @@ template <typename R> struct ContactRec {
   int flags;        // 1: rows enabled (collisionResponse of bodies and shapes, no trigger)
                     // 2: friction rows present
   int cm;           // contact-material index
+  int si, sj;       // ContactEquation.si / .sj: shape instances (heightfield pillars: the field)
   f3 ri, rj, ni;
   R restitution, slip;
 };
@@ -85,6 +91,7 @@ template <typename R> struct DevWorld {
   CHP_G(R) sleep_speed;
   // body-shape instances
   CHP_G(int) si_type, si_hull, si_hf, si_response, si_group, si_mask, si_material;
+  CHP_G(int) si_shape;   // the scene shape of each instance (a shared shape object is one)
   CHP_G(float) si_off, si_ori, si_half;
   CHP_G(R) si_radius, si_bsr;
   // hull table
@@ -126,6 +133,17 @@ template <typename R> struct DevWorld {
   CHP_G(int) sgm;
   CHP_G(R) sbr;
   CHP_G(int) counters;   // [0] status [1] pairs [2] contacts [3] rows [4] iterations [5] levels [6] staging
+  // contact events (null / 0 when off): the overlap keepers' sorted keys (body and shape sets,
+  // two of each: the previous step's and the current one, by parity), two sort buffers of kcap
+  // keys, scan scratch, the justTest overlaps of the step (bi, bj, si, sj) and the event buffer
+  CHP_G(unsigned long long) ev_keys;   // 4 * ev_kcap: body[0], body[1], shape[0], shape[1]
+  CHP_G(unsigned long long) ev_tmp;    // 2 * ev_kcap
+  CHP_G(int) ev_scan;                  // ev_kcap + 1
+  CHP_G(int4) ev_jt;                   // pair_cap
+  CHP_G(int) ev_ctl;                   // EV_* words
+  CHP_G(chp_event) ev_buf;             // ev_cap
+  int ev_mask, ev_cap, ev_kcap;
+  int nshapes;   // scene shapes (the range of si_shape)
 };
 
 }  // namespace chp

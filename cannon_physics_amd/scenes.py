@@ -424,3 +424,39 @@ def sleep_world(seed=21, broadphase="naive"):
     if broadphase == "sap":
         bp.autoDetectAxis()
     return w
+
+
+def events_world(seed=31, broadphase="naive"):
+    """Contact events (world_class.dart:543-610, 703-770): bodies that bounce (restitution), so
+    that contacts begin and end; spheres sharing one Sphere object (one shape key for all); a
+    compound body (box + sphere); a trigger; kinematic spheres crossing each other and a
+    kinematic box sliding through a static box and down through the plane (justTest pairs,
+    narrow_phase.dart:665-668, 712-716)."""
+    rng = np.random.default_rng(1_000_003 * 31 + seed)
+    bp = SAPBroadphase() if broadphase == "sap" else NaiveBroadphase()
+    w = World(gravity=(0, -9.82, 0), broadphase=bp, solver=GSSolver(iterations=10))
+    bouncy = Material(friction=0.3, restitution=0.8)
+    g = ground_plane()
+    g.material = bouncy
+    w.addBody(g)
+    shared = Sphere(0.25)
+    for k in range(6):
+        w.addBody(Body(mass=1.0, position=(-2.0 + 0.8 * k, rng.uniform(0.6, 1.6), rng.uniform(-0.4, 0.4)),
+                       material=bouncy, shape=shared))
+    for k in range(4):
+        w.addBody(Body(mass=1.0, position=(-1.5 + 1.0 * k, rng.uniform(1.0, 2.0), 1.2), material=bouncy,
+                       quaternion=shoemake_quaternion(rng), shape=Box(rng.uniform(0.15, 0.3, size=3))))
+    comp = Body(mass=2.0, position=(0.3, 1.4, -1.2), material=bouncy)
+    comp.addShape(Box((0.3, 0.15, 0.15)))
+    comp.addShape(Sphere(0.2), offset=(0.45, 0.0, 0.0))
+    w.addBody(comp)
+    w.addBody(Body(mass=1.0, position=(1.8, 0.35, -1.0), isTrigger=True, shape=Sphere(0.4)))
+    # justTest pairs: kinematic spheres crossing, a kinematic box through a static box
+    w.addBody(Body(mass=1.0, type=2, position=(-1.0, 2.5, -2.5), velocity=(1.1, 0.0, 0.0), shape=Sphere(0.3)))
+    w.addBody(Body(mass=1.0, type=2, position=(1.0, 2.5, -2.5), velocity=(-0.9, 0.0, 0.0), shape=Sphere(0.25)))
+    w.addBody(Body(mass=0.0, position=(0.0, 0.5, 2.6), shape=Box((0.3, 0.3, 0.3))))
+    w.addBody(Body(mass=1.0, type=2, position=(-1.2, 0.65, 2.6), velocity=(0.8, -0.15, 0.0),
+                   shape=Box((0.2, 0.2, 0.2))))
+    if broadphase == "sap":
+        bp.autoDetectAxis()
+    return w

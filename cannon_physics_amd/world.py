@@ -126,6 +126,18 @@ class Context:
         self.check(self.lib.chp_world_stats(self.ptr, wid, C.byref(s)))
         return {k: getattr(s, k) for k, _ in A.StepStats._fields_ if k != "reserved"}
 
+    def set_events(self, wid, mask=0x1f, capacity=0):
+        """records the event kinds in mask (bit A.EVENT_* set) from the next step on; 0 = off"""
+        self.check(self.lib.chp_world_set_events(self.ptr, wid, mask, capacity))
+
+    def events(self, wid):
+        """the events recorded since the last call, oldest first, as a structured numpy array"""
+        n = C.c_uint32()
+        self.check(self.lib.chp_world_events(self.ptr, wid, None, 0, C.byref(n)))
+        buf = (A.Event * max(1, n.value))()
+        self.check(self.lib.chp_world_events(self.ptr, wid, buf, n.value, C.byref(n)))
+        return np.array(np.ctypeslib.as_array(buf)[: n.value])
+
     def set_time(self, wid, t):
         """World.time of a world (the clock of the sleep timer)"""
         self.check(self.lib.chp_world_set_time(self.ptr, wid, float(t)))
@@ -168,11 +180,12 @@ class Context:
                                             C.byref(nc)))
         return out, int(neq.value), int(nf.value), int(nc.value)
 
-    def solve(self, wid, eqs, n, dt):
-        """Solver slot: GSSolver.solve on the device state; returns (iterations, multipliers)."""
+    def solve(self, wid, eqs, n, dt, iterations=-1, tolerance=-1.0):
+        """Solver slot: GSSolver.solve on the device state; returns (iterations, multipliers).
+        iterations / tolerance < 0: the world's GSSolver settings."""
         it = C.c_int32()
         mult = (C.c_double * max(1, n))()
-        self.check(self.lib.chp_solve(self.ptr, wid, eqs, n, dt, C.byref(it), mult))
+        self.check(self.lib.chp_solve(self.ptr, wid, eqs, n, dt, iterations, tolerance, C.byref(it), mult))
         return int(it.value), np.frombuffer(mult, dtype=np.float64, count=n).copy()
 
     # ---- serving ----

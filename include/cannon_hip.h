@@ -221,6 +221,9 @@ typedef struct chp_equation {
   double restitution;                    /* ContactEquation.restitution */
   double angle_cos;                      /* rotational / cone: cos(maxAngle) */
   double target_velocity;                /* RotationalMotorEquation.targetVelocity */
+  int32_t shape_i, shape_j;              /* contact: ContactEquation.si / .sj as scene shape
+                                            indices (heightfield pillars: the heightfield); -1
+                                            for the other kinds */
 } chp_equation;
 
 /* ---- context ----------------------------------------------------------------------------- */
@@ -305,6 +308,48 @@ typedef struct chp_step_stats {
 int chp_world_set_time(chp_ctx* ctx, uint32_t world_id, double time);
 int chp_world_stats(chp_ctx* ctx, uint32_t world_id, chp_step_stats* out);
 
+/* ---- contact events ---------------------------------------------------------------------- */
+/* The World's contact events (world_class.dart:543-610, 703-770):
+ *  - collide: one per contact of a step, in the step's contact order.  The reference dispatches
+ *    it twice, to contact.bi with body = bj and to contact.bj with body = bi
+ *    (collisionMatrixPrevious aliases collisionMatrix, so every contact raises it, every step);
+ *  - begin / endContact: body pairs that start / stop touching, from the body overlap keeper
+ *    (overlap_keeper.dart:21-78): keys of the step's contacts and of the justTest pairs
+ *    (kinematic against static or kinematic) whose resolver reports an overlap, sorted and
+ *    unique, diffed against the previous step's keys;
+ *  - begin / endShapeContact: the same over shape pairs.
+ * Within a step the events come in the reference's dispatch order: collide events, then
+ * beginContact, endContact, beginShapeContact, endShapeContact, each diff in ascending key
+ * order.  Bodies are world-local indices (the reference keys Body.id, which orders bodies like
+ * their indices when they are created in the order they are added).  Shapes are indices into
+ * the scene's shape table; a shared shape object is one shape, as in the reference.  A
+ * convex–heightfield contact names the heightfield (the reference names the internal pillar
+ * hull, a temporary with no stable id). */
+enum { CHP_EVENT_COLLIDE = 0, CHP_EVENT_BEGIN_CONTACT = 1, CHP_EVENT_END_CONTACT = 2,
+       CHP_EVENT_BEGIN_SHAPE_CONTACT = 3, CHP_EVENT_END_SHAPE_CONTACT = 4 };
+typedef struct chp_event {
+  uint32_t step;                         /* the world's step counter during the raising step */
+  int32_t kind;                          /* CHP_EVENT_* */
+  int32_t body_a, body_b;                /* collide: contact.bi, contact.bj; others: the pair */
+  int32_t shape_a, shape_b;              /* collide: contact.si, contact.sj; shape events: the
+                                            pair; body events: -1 */
+  float ni[3], ri[3], rj[3];             /* collide: the contact's normal and contact points
+                                            (relative to the bodies' centres); 0 otherwise */
+  int32_t reserved;
+} chp_event;
+/* Records the event kinds in kind_mask (bit CHP_EVENT_* set = recorded) on the device, into a
+ * buffer of `capacity` events per world (0: a default of 4 per contact slot).  The overlap
+ * keepers run whenever the mask is not 0; their first set is the contact list of the last step
+ * taken (justTest overlaps of that step are not known).  kind_mask 0 stops the bookkeeping and
+ * frees its buffers. */
+int chp_world_set_events(chp_ctx* ctx, uint32_t world_id, uint32_t kind_mask, uint32_t capacity);
+/* Moves the events recorded since the last call (across every step taken) into out, oldest
+ * first; *n receives their number.  out == NULL: *n receives the number stored, nothing moves.
+ * CHP_E_CAPACITY when cap is too small (nothing is moved, *n = the number stored) or when the
+ * device buffer overflowed during the steps (the events are dropped, *n = the number raised:
+ * the capacity chp_world_set_events needs for that many steps). */
+int chp_world_events(chp_ctx* ctx, uint32_t world_id, chp_event* out, uint32_t cap, uint32_t* n);
+
 /* ---- slot-level entry points (one world, current device state) --------------------------- */
 /* Broadphase.collisionPairs: pairs in reference emission order (world-local body indices). */
 int chp_broadphase(chp_ctx* ctx, uint32_t world_id, int32_t* pi, int32_t* pj, uint32_t cap,
@@ -328,11 +373,12 @@ int chp_narrowphase(chp_ctx* ctx, uint32_t world_id, const int32_t* pi, const in
 /* GSSolver.solve(dt, world) (gs_solver.dart:27-133) on the world's current device state for the
  * given equations in solver order: B and 1/C from each equation's own spook parameters and the
  * bodies' applied force as stored (the World adds gravity before calling its solver), the
- * world's iterations and tolerance, then velocity += vlambda * linearFactor and
+ * calling solver's iterations and tolerance (GSSolver.iterations / .tolerance; < 0: the world's
+ * own, from chp_world_desc), then velocity += vlambda * linearFactor and
  * angularVelocity += wlambda * angularFactor.  iterations receives GSSolver.solve's return
  * value; multipliers (may be NULL) receive Equation.multiplier = lambda / dt. */
 int chp_solve(chp_ctx* ctx, uint32_t world_id, const chp_equation* eqs, uint32_t n, double dt,
-              int32_t* iterations, double* multipliers);
+              int32_t max_iterations, double tolerance, int32_t* iterations, double* multipliers);
 
 /* ---- serving: continuous batching of requests --------------------------------------------- */
 /* A request is a world and a number of steps.  chp_serve_submit replays its constructors on the

@@ -81,6 +81,17 @@ void orc_step(void* wp, double dt, int n) {
   }
 }
 double orc_time(void* wp) { return ((World*)wp)->time; }
+/* contact events: keepers and the event list run from the next step on */
+void orc_set_events(void* wp, int on) { ((World*)wp)->eventsOn = on != 0; }
+/* moves the recorded events out when cap holds them all; returns how many there are */
+int orc_events(void* wp, chp_event* out, int cap) {
+  World& w = *(World*)wp;
+  int n = (int)w.events.size();
+  if (!out || cap < n) return n;
+  for (int k = 0; k < n; k++) out[k] = w.events[k];
+  w.events.clear();
+  return n;
+}
 void orc_set_time(void* wp, double t) { ((World*)wp)->time = t; }
 
 /* steps many worlds, one std::thread per world slice (the "one isolate per world" analogue) */
@@ -156,6 +167,8 @@ int orc_narrowphase(void* wp, const int* pi, const int* pj, int n, double dt, ch
     chp_equation& o = out[k];
     memset(&o, 0, sizeof o);
     o.kind = e.kind; o.body_i = e.bi->index; o.body_j = e.bj->index;
+    o.shape_i = e.kind == CHP_ROW_CONTACT ? e.si : -1;   /* FrictionEquation has no shapes */
+    o.shape_j = e.kind == CHP_ROW_CONTACT ? e.sj : -1;
     put3(o.ri, e.ri); put3(o.rj, e.rj);
     put3(o.n, e.kind == CHP_ROW_FRICTION ? e.t : e.ni);
     o.min_force = (double)e.minForce; o.max_force = (double)e.maxForce;
@@ -176,9 +189,15 @@ int orc_sat(void* wp, int a, int b, int face0_only, float* axis, double* depth) 
   return r;
 }
 
-/* Solver slot: GSSolver.solve on the current state for the given equations */
-int orc_solve(void* wp, const chp_equation* in, int n, double dt, double* multipliers) {
+/* Solver slot: GSSolver.solve on the current state for the given equations; iterations /
+ * tolerance < 0: the world's solver settings */
+int orc_solve(void* wp, const chp_equation* in, int n, double dt, int iterations, double tolerance,
+              double* multipliers) {
   World& w = *(World*)wp;
+  const int it0 = w.iterations;
+  const R tol0 = w.tolerance;
+  if (iterations >= 0) w.iterations = iterations;
+  if (tolerance >= 0) w.tolerance = (R)tolerance;
   std::vector<Equation> eqs(n);
   for (int k = 0; k < n; k++) {
     const chp_equation& e = in[k];
@@ -199,6 +218,8 @@ int orc_solve(void* wp, const chp_equation* in, int n, double dt, double* multip
   }
   std::vector<R> mult;
   int it = orc::solve_equations(w, eqs, (R)dt, mult);
+  w.iterations = it0;
+  w.tolerance = tol0;
   if (multipliers)
     for (int k = 0; k < n; k++) multipliers[k] = (double)mult[k];
   return it;

@@ -440,6 +440,7 @@ World* build_world(const chp_scene* sc) {
   for (uint32_t s = 0; s < sc->num_shapes; s++) {
     const chp_shape_desc& d = sc->shapes[s];
     std::unique_ptr<Shape> sh(new Shape());
+    sh->id = (int)s;
     sh->type = d.type;
     sh->collisionResponse = d.collision_response != 0;
     sh->group = d.collision_filter_group;
@@ -511,6 +512,9 @@ World* build_world(const chp_scene* sc) {
     else w->sapAxis = 2;
   }
   for (uint32_t c = 0; c < sc->num_constraints; c++) build_constraint(*w, sc->constraints[c]);
+  w->shapeBody.assign(sc->num_shapes, -1);   /* Body.addShape sets shape.body: the last body wins */
+  for (Body* b : w->bodies)
+    for (Shape* s : b->shapes) w->shapeBody[s->id] = b->index;
   return w;
 }
 

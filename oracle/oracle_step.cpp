@@ -90,16 +90,20 @@ void broadphase_pairs(World& w, std::vector<Body*>& p1, std::vector<Body*>& p2) 
 /* narrowphase                                                                                */
 /* ------------------------------------------------------------------------------------------ */
 
-struct SRef { bool cr; int material; };     /* the Shape fields createContactEquation reads */
+struct SRef { bool cr; int material; int id; };   /* the Shape fields createContactEquation reads */
+
+struct JustTestHit {};   /* a justTest resolver reached its first contact: it returns true */
 
 struct NP {
   World& w;
+  bool justTest = false;                    /* the resolvers' justTest argument */
   ContactMat cm;                            /* currentContactMaterial */
   V3 sepAxis;                               /* _convexConvexSepAxis persists across calls */
   explicit NP(World& w_) : w(w_) {}
 
   /* Narrowphase.createContactEquation, narrow_phase.dart:492-528 */
   Equation& createContact(Body* bi, Body* bj, SRef si, SRef sj, const SRef* rsi, const SRef* rsj) {
+    if (justTest) throw JustTestHit();   /* `if (justTest) return true;` before each contact */
     w.contacts.emplace_back();
     Equation& c = w.contacts.back();
     c.kind = CHP_ROW_CONTACT;
@@ -114,6 +118,8 @@ struct NP {
       c.restitution = w.materials[matA].restitution * w.materials[matB].restitution;
     c.shapeMatA = (rsi ? *rsi : si).material;
     c.shapeMatB = (rsj ? *rsj : sj).material;
+    c.si = (rsi ? *rsi : si).id;   /* c.si = overrideShapeA ?? si */
+    c.sj = (rsj ? *rsj : sj).id;
     return c;
   }
   /* Equation.setSpookParams, equation_class.dart:53-60 */
@@ -158,7 +164,7 @@ struct NP {
   /* sphereSphere, narrow_phase.dart:723-765 */
   void sphereSphere(const Shape& si, const Shape& sj, const V3& xi, const V3& xj, Body* bi, Body* bj,
                     const SRef* rsi, const SRef* rsj) {
-    Equation& c = createContact(bi, bj, {si.collisionResponse, si.material}, {sj.collisionResponse, sj.material}, rsi, rsj);
+    Equation& c = createContact(bi, bj, {si.collisionResponse, si.material, si.id}, {sj.collisionResponse, sj.material, sj.id}, rsi, rsj);
     size_t ci = w.contacts.size() - 1;
     sub2(xj, xi, c.ni);
     normalize(c.ni);
@@ -175,7 +181,7 @@ struct NP {
                    Body* bi, Body* bj, const SRef* rsi, const SRef* rsj) {
     Equation r = Equation();
     /* the contact object is taken before the test; only appended when touching */
-    SRef a{si.collisionResponse, si.material}, b{sj.collisionResponse, sj.material};
+    SRef a{si.collisionResponse, si.material, si.id}, b{sj.collisionResponse, sj.material, sj.id};
     setValues(r.ni, 0, 0, 1);
     vmult(qj, r.ni, r.ni);
     negate(r.ni);
@@ -198,7 +204,7 @@ struct NP {
   /* sphereBox, narrow_phase.dart:816-1038 (+ Box.getSideNormals, box.dart:99-115) */
   void sphereBox(const Shape& si, const Shape& sj, const V3& xi, const V3& xj, const Q4& qj,
                  Body* bi, Body* bj, const SRef* rsi, const SRef* rsj) {
-    SRef A{si.collisionResponse, si.material}, B{sj.collisionResponse, sj.material};
+    SRef A{si.collisionResponse, si.material, si.id}, B{sj.collisionResponse, sj.material, sj.id};
     V3 sides[6];
     const V3& ex = sj.halfExtents;
     setValues(sides[0], L(ex.x), 0, 0); setValues(sides[1], 0, L(ex.y), 0); setValues(sides[2], 0, 0, L(ex.z));
@@ -336,7 +342,7 @@ struct NP {
   /* sphereConvex, narrow_phase.dart:1039-1257 */
   void sphereConvex(const Shape& si, const Hull& sj, SRef sjRef, const V3& xi, const V3& xj,
                     const Q4& qj, Body* bi, Body* bj, const SRef* rsi, const SRef* rsj) {
-    SRef A{si.collisionResponse, si.material};
+    SRef A{si.collisionResponse, si.material, si.id};
     V3 convexToSphere;
     sub2(xi, xj, convexToSphere);
     R Rr = si.radius;
@@ -431,7 +437,7 @@ struct NP {
   /* planeConvex, narrow_phase.dart:1847-1915 */
   void planeConvex(const Shape& si, const Hull& sj, SRef sjRef, const V3& xi, const V3& xj,
                    const Q4& qi, const Q4& qj, Body* bi, Body* bj, const SRef* rsi, const SRef* rsj) {
-    SRef A{si.collisionResponse, si.material};
+    SRef A{si.collisionResponse, si.material, si.id};
     V3 worldNormal;
     setValues(worldNormal, 0, 0, 1);
     vmult(qi, worldNormal, worldNormal);
@@ -730,10 +736,10 @@ struct NP {
     R mn, mx;
     rectMinMax(hf, iMinX, iMinY, iMaxX, iMaxY, mn, mx);
     if (L(local.z) - radius > mx || L(local.z) + radius < mn) return;
-    SRef pillarRef{true, -1};
+    SRef pillarRef{true, -1, sj.id};   /* the pillar hull: its contacts name the heightfield */
     /* the call below passes (si, sj) = (sphere, heightfield) as the override shapes, whatever
      * overrides sphereHeightfield itself received */
-    SRef oa{si.collisionResponse, si.material}, ob{sj.collisionResponse, sj.material};
+    SRef oa{si.collisionResponse, si.material, si.id}, ob{sj.collisionResponse, sj.material, sj.id};
     (void)rsi; (void)rsj;
     for (int i = iMinX; i < iMaxX; i++)
       for (int j = iMinY; j < iMaxY; j++) {
@@ -762,7 +768,7 @@ struct NP {
     rectMinMax(hf, iMinX, iMinY, iMaxX, iMaxY, mn, mx);
     if (L(local.z) - radius > mx || L(local.z) + radius < mn) return;
     static const std::vector<int> faceList = {0};
-    SRef pillarRef{true, -1};
+    SRef pillarRef{true, -1, sj.id};   /* the pillar hull: its contacts name the heightfield */
     for (int i = iMinX; i < iMaxX; i++)
       for (int j = iMinY; j < iMaxY; j++)
         for (int up = 0; up < 2; up++) {
@@ -801,7 +807,29 @@ struct NP {
           const ContactMat* shapeCM = nullptr;
           if (si.material >= 0 && sj.material >= 0) shapeCM = lookup(si.material, sj.material);
           cm = shapeCM ? *shapeCM : (bodyCM ? *bodyCM : w.defaultCM);
-          if (justTest) continue;  /* justTest resolvers create no contacts */
+          if (justTest) {   /* justTest resolvers create no contacts; an overlap is registered */
+            if (!w.eventsOn) continue;
+            bool hit = false;
+            if (si.type == CHP_SHAPE_SPHERE && sj.type == CHP_SHAPE_SPHERE) {  /* :737-739 */
+              R dx = L(xi.x) - L(xj.x), dy = L(xi.y) - L(xj.y), dz = L(xi.z) - L(xj.z);
+              R rr = si.radius + sj.radius;
+              hit = dx * dx + dy * dy + dz * dz < rr * rr;
+            } else {
+              this->justTest = true;
+              try {
+                if (si.type < sj.type) dispatch(si, sj, xi, xj, qi, qj, bi, bj, si, sj);
+                else dispatch(sj, si, xj, xi, qj, qi, bj, bi, si, sj);
+              } catch (const JustTestHit&) {
+                hit = true;
+              }
+              this->justTest = false;
+            }
+            if (hit) {  /* :712-716 */
+              w.shapeOverlapKeeper.set(si.id, sj.id);
+              w.bodyOverlapKeeper.set(bi->index, bj->index);
+            }
+            continue;
+          }
           if (si.type < sj.type)
             dispatch(si, sj, xi, xj, qi, qj, bi, bj, si, sj);
           else
@@ -818,31 +846,31 @@ struct NP {
    * the hot path covers; A.type <= B.type on entry */
   void dispatch(const Shape& A, const Shape& B, const V3& xa, const V3& xb, const Q4& qa, const Q4& qb,
                 Body* ba, Body* bb, const Shape& rsiS, const Shape& rsjS) {
-    SRef rsi{rsiS.collisionResponse, rsiS.material}, rsj{rsjS.collisionResponse, rsjS.material};
+    SRef rsi{rsiS.collisionResponse, rsiS.material, rsiS.id}, rsj{rsjS.collisionResponse, rsjS.material, rsjS.id};
     int ta = A.type, tb = B.type;
     if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_SPHERE) sphereSphere(A, B, xa, xb, ba, bb, &rsi, &rsj);
     else if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_PLANE) spherePlane(A, B, xa, xb, qb, ba, bb, &rsi, &rsj);
     else if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_BOX) sphereBox(A, B, xa, xb, qb, ba, bb, &rsi, &rsj);
     else if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_CONVEX)
-      sphereConvex(A, *B.hull, SRef{B.collisionResponse, B.material}, xa, xb, qb, ba, bb, &rsi, &rsj);
+      sphereConvex(A, *B.hull, SRef{B.collisionResponse, B.material, B.id}, xa, xb, qb, ba, bb, &rsi, &rsj);
     else if (ta == CHP_SHAPE_SPHERE && tb == CHP_SHAPE_HEIGHTFIELD) sphereHeightfield(A, B, xa, xb, qb, ba, bb, &rsi, &rsj);
     else if (ta == CHP_SHAPE_PLANE && tb == CHP_SHAPE_BOX)  /* planeBox :1786-1804 */
-      planeConvex(A, B.repr, SRef{B.collisionResponse, B.material}, xa, xb, qa, qb, ba, bb, &rsi, &rsj);
+      planeConvex(A, B.repr, SRef{B.collisionResponse, B.material, B.id}, xa, xb, qa, qb, ba, bb, &rsi, &rsj);
     else if (ta == CHP_SHAPE_PLANE && tb == CHP_SHAPE_CONVEX)
-      planeConvex(A, *B.hull, SRef{B.collisionResponse, B.material}, xa, xb, qa, qb, ba, bb, &rsi, &rsj);
+      planeConvex(A, *B.hull, SRef{B.collisionResponse, B.material, B.id}, xa, xb, qa, qb, ba, bb, &rsi, &rsj);
     else if (ta == CHP_SHAPE_BOX && tb == CHP_SHAPE_BOX)  /* boxBox :1693-1712 */
-      convexConvex(A.repr, SRef{A.collisionResponse, A.material}, B.repr, SRef{B.collisionResponse, B.material},
+      convexConvex(A.repr, SRef{A.collisionResponse, A.material, A.id}, B.repr, SRef{B.collisionResponse, B.material, B.id},
                    xa, xb, qa, qb, ba, bb, &rsi, &rsj, nullptr);
     else if (ta == CHP_SHAPE_BOX && tb == CHP_SHAPE_CONVEX)  /* boxConvex :1713-1730 */
-      convexConvex(A.repr, SRef{A.collisionResponse, A.material}, *B.hull, SRef{B.collisionResponse, B.material},
+      convexConvex(A.repr, SRef{A.collisionResponse, A.material, A.id}, *B.hull, SRef{B.collisionResponse, B.material, B.id},
                    xa, xb, qa, qb, ba, bb, &rsi, &rsj, nullptr);
     else if (ta == CHP_SHAPE_CONVEX && tb == CHP_SHAPE_CONVEX)
-      convexConvex(*A.hull, SRef{A.collisionResponse, A.material}, *B.hull, SRef{B.collisionResponse, B.material},
+      convexConvex(*A.hull, SRef{A.collisionResponse, A.material, A.id}, *B.hull, SRef{B.collisionResponse, B.material, B.id},
                    xa, xb, qa, qb, ba, bb, &rsi, &rsj, nullptr);
     else if (ta == CHP_SHAPE_BOX && tb == CHP_SHAPE_HEIGHTFIELD)  /* boxHeightfield :1749-1766 */
-      heightfieldConvex(A.repr, SRef{A.collisionResponse, A.material}, B, xa, xb, qa, qb, ba, bb);
+      heightfieldConvex(A.repr, SRef{A.collisionResponse, A.material, A.id}, B, xa, xb, qa, qb, ba, bb);
     else if (ta == CHP_SHAPE_CONVEX && tb == CHP_SHAPE_HEIGHTFIELD)
-      heightfieldConvex(*A.hull, SRef{A.collisionResponse, A.material}, B, xa, xb, qa, qb, ba, bb);
+      heightfieldConvex(*A.hull, SRef{A.collisionResponse, A.material, A.id}, B, xa, xb, qa, qb, ba, bb);
     /* other combinations have no resolver: skipped like narrow_phase.dart:699-700 */
   }
 };
@@ -1154,6 +1182,10 @@ void internal_step(World& w, R dt) {
       }
     }
   }
+  if (w.eventsOn) {  /* collisionMatrixTick, :212-220 */
+    w.bodyOverlapKeeper.tick();
+    w.shapeOverlapKeeper.tick();
+  }
   w.cap.pi.clear(); w.cap.pj.clear();
   for (size_t k = 0; k < p1.size(); k++) { w.cap.pi.push_back(p1[k]->index); w.cap.pj.push_back(p2[k]->index); }
   w.contacts.clear();
@@ -1173,6 +1205,20 @@ void internal_step(World& w, R dt) {
       if (ma.restitution >= 0 && mb.restitution >= 0) c.restitution = ma.restitution * mb.restitution;
     }
     add(&c);
+    if (w.eventsOn) {  /* collide event (collisionMatrixPrevious aliases collisionMatrix), :592-607 */
+      chp_event e;
+      std::memset(&e, 0, sizeof e);
+      e.step = (uint32_t)w.stepnumber;
+      e.kind = CHP_EVENT_COLLIDE;
+      e.body_a = c.bi->index; e.body_b = c.bj->index;
+      e.shape_a = c.si; e.shape_b = c.sj;
+      const V3* v[3] = {&c.ni, &c.ri, &c.rj};
+      float* o[3] = {e.ni, e.ri, e.rj};
+      for (int k = 0; k < 3; k++) { o[k][0] = v[k]->x; o[k][1] = v[k]->y; o[k][2] = v[k]->z; }
+      w.events.push_back(e);
+      w.bodyOverlapKeeper.set(c.bi->index, c.bj->index);
+      w.shapeOverlapKeeper.set(c.si, c.sj);
+    }
     /* wake-up marks, :562-590 */
     Body* bi = c.bi;
     Body* bj = c.bj;
@@ -1187,6 +1233,27 @@ void internal_step(World& w, R dt) {
       R speedSquaredA = length2(bi->velocity) + length2(bi->angularVelocity);
       R speedLimitSquaredA = bi->sleepSpeedLimit * bi->sleepSpeedLimit;
       if (speedSquaredA >= speedLimitSquaredA * (R)2) bj->wakeUpAfterNarrowphase = true;
+    }
+  }
+  if (w.eventsOn) {  /* emitContactEvents, :703-770 */
+    for (int set = 0; set < 2; set++) {
+      std::vector<unsigned long long> additions, removals;
+      (set == 0 ? w.bodyOverlapKeeper : w.shapeOverlapKeeper).getDiff(additions, removals);
+      for (int dir = 0; dir < 2; dir++)
+        for (unsigned long long k : dir == 0 ? additions : removals) {
+          chp_event e;
+          std::memset(&e, 0, sizeof e);
+          e.step = (uint32_t)w.stepnumber;
+          e.kind = set == 0 ? (dir == 0 ? CHP_EVENT_BEGIN_CONTACT : CHP_EVENT_END_CONTACT)
+                            : (dir == 0 ? CHP_EVENT_BEGIN_SHAPE_CONTACT : CHP_EVENT_END_SHAPE_CONTACT);
+          int a = (int)(k >> 32), b = (int)(k & 0xffffffffull);
+          if (set == 0) { e.body_a = a; e.body_b = b; e.shape_a = e.shape_b = -1; }
+          else {  /* Shape.body */
+            e.shape_a = a; e.shape_b = b;
+            e.body_a = w.shapeBody[a]; e.body_b = w.shapeBody[b];
+          }
+          w.events.push_back(e);
+        }
     }
   }
   for (Body* b : w.bodies)  /* wake up bodies, :617-624 */

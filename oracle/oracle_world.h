@@ -34,6 +34,7 @@ struct Heightfield {             /* lib/rigid_body_shapes/heightfield.dart */
 };
 
 struct Shape {
+  int id = -1;                   /* the scene's shape index (Shape.id) */
   int type = CHP_SHAPE_SPHERE;
   R boundingSphereRadius = 0;
   bool collisionResponse = true;
@@ -102,6 +103,7 @@ struct Equation {
   R targetVelocity = 0;
   /* events only: shapes recorded on the contact */
   int shapeMatA = -1, shapeMatB = -1;   /* (c.si.material, c.sj.material) for friction */
+  int si = -1, sj = -1;                 /* ContactEquation.si / .sj (scene shape indices) */
 };
 
 struct Constraint {
@@ -132,6 +134,37 @@ struct StepCapture {
   int contacts = 0;
 };
 
+/* OverlapKeeper (overlap_keeper.dart:1-78): sorted, duplicate-free pair keys of this step and
+ * the previous one; the key is (min << 32) | max (the reference packs 16 bits per index) */
+struct OverlapKeeper {
+  std::vector<unsigned long long> current, previous;
+  static unsigned long long key(int i, int j) {
+    if (j < i) std::swap(i, j);
+    return ((unsigned long long)(unsigned)i << 32) | (unsigned)j;
+  }
+  void set(int i, int j) {   /* insertion sort, :22-38 */
+    unsigned long long k = key(i, j);
+    size_t index = 0;
+    while (index < current.size() && k > current[index]) index++;
+    if (index < current.size() && current[index] == k) return;
+    current.insert(current.begin() + (long)index, k);
+  }
+  void tick() { previous.swap(current); current.clear(); }
+  /* getDiff, :46-75 */
+  void getDiff(std::vector<unsigned long long>& additions, std::vector<unsigned long long>& removals) const {
+    size_t j = 0;
+    for (size_t i = 0; i < current.size(); i++) {
+      while (j < previous.size() && current[i] > previous[j]) j++;
+      if (!(j < previous.size() && current[i] == previous[j])) additions.push_back(current[i]);
+    }
+    j = 0;
+    for (size_t i = 0; i < previous.size(); i++) {
+      while (j < current.size() && previous[i] > current[j]) j++;
+      if (!(j < current.size() && previous[i] == current[j])) removals.push_back(previous[i]);
+    }
+  }
+};
+
 struct World {
   V3 gravity;
   bool hasFrictionGravity = false;
@@ -157,6 +190,11 @@ struct World {
   std::vector<std::unique_ptr<Hull>> hullStore;
   std::vector<std::unique_ptr<Heightfield>> hfStore;
   std::vector<std::unique_ptr<Constraint>> constraints;
+  /* contact events (world_class.dart:543-610, 703-770), kept only while eventsOn */
+  bool eventsOn = false;
+  OverlapKeeper bodyOverlapKeeper, shapeOverlapKeeper;
+  std::vector<chp_event> events;
+  std::vector<int> shapeBody;   /* Shape.body per scene shape */
   /* per-step scratch */
   std::vector<Equation> contacts;
   std::vector<Equation> frictions;

@@ -44,9 +44,11 @@ def lib(mode="f64"):
         "orc_capture_contact_points": (C.c_int, [vp, P(C.c_int), P(C.c_float), C.c_int]),
         "orc_narrowphase": (C.c_int, [vp, P(C.c_int), P(C.c_int), C.c_int, C.c_double, P(A.Equation), C.c_int,
                                       P(C.c_int), P(C.c_int)]),
-        "orc_solve": (C.c_int, [vp, P(A.Equation), C.c_int, C.c_double, P(C.c_double)]),
+        "orc_solve": (C.c_int, [vp, P(A.Equation), C.c_int, C.c_double, C.c_int, C.c_double, P(C.c_double)]),
         "orc_sat": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_double)]),
         "orc_time": (C.c_double, [vp]),
+        "orc_set_events": (None, [vp, C.c_int]),
+        "orc_events": (C.c_int, [vp, P(A.Event), C.c_int]),
         "orc_set_time": (None, [vp, C.c_double]),
     }
     for name, (res, args) in sig.items():
@@ -126,6 +128,17 @@ class OracleWorld:
     def contacts(self):
         return self.L.orc_capture_contacts(self.ptr)
 
+    def set_events(self, on=True):
+        """keep the overlap keepers and record the contact events from the next step on"""
+        self.L.orc_set_events(self.ptr, int(on))
+
+    def events(self):
+        """the events recorded since the last call, oldest first (structured numpy array)"""
+        n = self.L.orc_events(self.ptr, None, 0)
+        buf = (A.Event * max(1, n))()
+        self.L.orc_events(self.ptr, buf, n)
+        return np.array(np.ctypeslib.as_array(buf)[:n])
+
     def time(self):
         """World.time"""
         return self.L.orc_time(self.ptr)
@@ -166,8 +179,8 @@ class OracleWorld:
                         "rj": np.array(e.rj[:], dtype=np.float64), "n": np.array(e.n[:], dtype=np.float64)})
         return res
 
-    def solve(self, eqs, n, dt):
+    def solve(self, eqs, n, dt, iterations=-1, tolerance=-1.0):
         """Solver slot on the current state: (iterations, multipliers)"""
         mult = (C.c_double * max(1, n))()
-        it = self.L.orc_solve(self.ptr, eqs, n, dt, mult)
+        it = self.L.orc_solve(self.ptr, eqs, n, dt, iterations, tolerance, mult)
         return it, np.frombuffer(mult, dtype=np.float64, count=n).copy()

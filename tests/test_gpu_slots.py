@@ -81,6 +81,17 @@ def test_narrowphase_and_solver_slots_match_oracle(name, mode):
         assert it == oit
         assert np.array_equal(bits(mult), bits(omult))
         assert same_state(c, wid, ow)
+        # the calling solver's own iterations and tolerance (GSSolver.iterations / .tolerance)
+        for iters, tol in [(3, -1.0), (25, 1e-3), (-1, 0.5)]:
+            it, mult = c.solve(wid, eqs, k, DT, iterations=iters, tolerance=tol)
+            oit, omult = ow.solve(eqs, k, DT, iterations=iters, tolerance=tol)
+            assert it == oit and (iters < 0 or it <= iters)
+            assert np.array_equal(bits(mult), bits(omult))
+            assert same_state(c, wid, ow)
+        # the world's own settings are back for the next call
+        it, mult = c.solve(wid, eqs, k, DT)
+        oit, omult = ow.solve(eqs, k, DT)
+        assert it == oit and np.array_equal(bits(mult), bits(omult))
     finally:
         c.destroy_world(wid)
 
