@@ -275,6 +275,36 @@ class Event(C.Structure):
     ]
 
 
+RAY_CLOSEST, RAY_ANY, RAY_ALL = 0, 1, 2
+
+
+class Ray(C.Structure):
+    """chp_ray: one ray of World.raycastClosest / Any / All"""
+    _fields_ = [
+        ("from_", F3),
+        ("to", F3),
+        ("mode", C.c_int32),
+        ("skip_backfaces", C.c_int32),
+        ("check_collision_response", C.c_int32),
+        ("collision_filter_group", C.c_int32),
+        ("collision_filter_mask", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class RayHit(C.Structure):
+    """chp_ray_hit: a RaycastResult"""
+    _fields_ = [
+        ("ray", C.c_int32),
+        ("body", C.c_int32),
+        ("shape", C.c_int32),
+        ("face", C.c_int32),
+        ("normal", F3),
+        ("point", F3),
+        ("distance", C.c_double),
+    ]
+
+
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
 
 
@@ -327,4 +357,6 @@ STRUCT_SIZES = {
     "chp_equation": C.sizeof(Equation),
     "chp_request_status": C.sizeof(RequestStatus),
     "chp_event": C.sizeof(Event),
+    "chp_ray": C.sizeof(Ray),
+    "chp_ray_hit": C.sizeof(RayHit),
 }

@@ -57,6 +57,10 @@ def load():
         "chp_world_stats": (i32, [ctx_p, u32, P(A.StepStats)]),
         "chp_world_set_time": (i32, [ctx_p, u32, dbl]),
         "chp_world_set_events": (i32, [ctx_p, u32, u32, u32]),
+        "chp_world_step_interpolated": (i32, [ctx_p, u32, dbl, dbl, i32, P(i32)]),
+        "chp_interpolated_state": (i32, [ctx_p, u32, P(C.c_float), P(C.c_float), u32]),
+        "chp_aabb_query": (i32, [ctx_p, u32, P(C.c_float), P(C.c_float), P(i32), u32, P(u32)]),
+        "chp_raycast": (i32, [ctx_p, u32, P(A.Ray), u32, P(A.RayHit), u32, P(u32)]),
         "chp_world_events": (i32, [ctx_p, u32, P(A.Event), u32, P(u32)]),
         "chp_broadphase": (i32, [ctx_p, u32, P(i32), P(i32), u32, P(u32)]),
         "chp_step_capture": (i32, [ctx_p, u32, dbl, P(i32), P(i32), u32, P(u32), P(A.Row), u32,

@@ -18,6 +18,7 @@
 #include "chp_build.h"
 #include "chp_stages.h"
 #include "chp_events.h"
+#include "chp_query.h"
 
 using namespace chp;
 
@@ -381,6 +382,44 @@ __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worl
   }
 }
 
+// World.step(dt, timeSinceLastCalled): the interpolated poses and World.time += the elapsed time
+template <typename R>
+__global__ __launch_bounds__(256) void k_interp(const DevWorld<R>* __restrict__ worlds, int id, double t, double elapsed) {
+  const DevWorld<R>& W = worlds[id];
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i == 0) W.wtime[0] += elapsed;
+  if (i < W.nb) interp_body(W, i, (R)t);
+}
+
+// queries: the preparation (AABBs, SAP axis list), the AABB query, the ray passes
+template <typename R>
+__global__ __launch_bounds__(1024) void k_query_prep(const DevWorld<R>* __restrict__ worlds, int id) {
+  st_query_prep(worlds[id]);
+}
+template <typename R>
+__global__ __launch_bounds__(1024) void k_aabb_query(const DevWorld<R>* __restrict__ worlds, int id, f3 lo, f3 hi,
+                                                     int* __restrict__ out, int cap, int* __restrict__ total) {
+  __shared__ int sh[1024];
+  const DevWorld<R>& W = worlds[id];
+  int carry = 0;
+  for (int k0 = 0; k0 < W.nb; k0 += blockDim.x) {
+    const int k = k0 + (int)threadIdx.x;
+    const int b = k < W.nb ? query_body(W, k) : 0;
+    const int hit = (k < W.nb && aabb_overlaps(F3<R>(W.aabb_lo, b), F3<R>(W.aabb_hi, b), lo, hi)) ? 1 : 0;
+    int t;
+    const int r = tile_exscan(hit, sh, &t);
+    if (hit && carry + r < cap) out[carry + r] = b;
+    carry += t;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+template <typename R, int PASS>
+__global__ __launch_bounds__(64) void k_rays(const DevWorld<R>* __restrict__ worlds, int id, const chp_ray* __restrict__ rays,
+                                             int nr, int4* __restrict__ probe, int* __restrict__ cnt,
+                                             const int* __restrict__ off, chp_ray_hit* __restrict__ out) {
+  st_rays<R, PASS>(worlds[id], rays, nr, probe, cnt, off, out);
+}
+
 // contact-event bookkeeping after the narrowphase (worlds with events on); seed: only the key
 // sets, from the contacts the slab holds
 template <typename R>
@@ -556,6 +595,7 @@ struct WorldSlot {
   size_t bytes = 0;
   double last_dt = -1;
   bool may_sleep = false;  // World.allowSleep, or a body asleep: the step runs the wake-up pass
+  double accumulator = 0;  // World.accumulator of World.step(dt, timeSinceLastCalled)
   // contact events: one device allocation (keys, sort buffers, scan, justTest list, control
   // words, events) while a kind mask is set
   struct Events {
@@ -638,6 +678,8 @@ struct chp_ctx {
   size_t max_lds = 65536;        // LDS bytes a workgroup may allocate on the device
   chp_equation* d_eq = nullptr;  // staging of the narrowphase / solver slots
   size_t d_eq_cap = 0;
+  void* d_qbuf = nullptr;        // staging of the queries (rays, hits, counts)
+  size_t d_qbuf_cap = 0;
   // serving
   struct Request {
     int state = CHP_REQ_QUEUED, error = CHP_OK;
@@ -729,6 +771,9 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   fix.emplace_back((void**)(void*)&D.adamp, o_ad);
   ARR(type, H.type); ARR(flags, H.flags); ARR(group, H.group); ARR(mask, H.mask); ARR(material, H.material);
   ARR(shape_first, H.shape_first); ARR(shape_count, H.shape_count); ARR(order, H.order);
+  ARR(ray_state, H.ray_state);
+  ARR(prev_pos, H.prev_pos); ARR(prev_quat, H.prev_quat); ARR(interp_pos, H.interp_pos);
+  ARR(interp_quat, H.interp_quat);
   ARR(sleep_state, H.sleep_state); ARR(sleep_tls, H.sleep_tls); ARR(sleep_speed, H.sleep_speed);
   ARR(sleep_time, H.sleep_time);
   {
@@ -1537,7 +1582,7 @@ template <typename R> static int set_capacity_t(chp_ctx* ctx, uint32_t id, uint3
   hipError_t e = hipSuccess;
 #define G(field) if (e == hipSuccess) e = get(H.field, D.field)
   G(pos); G(quat); G(vel); G(avel); G(force); G(torque); G(iiw); G(aabb_lo); G(aabb_hi); G(flags); G(order);
-  G(sleep_state); G(sleep_tls);
+  G(sleep_state); G(sleep_tls); G(ray_state); G(prev_pos); G(prev_quat); G(interp_pos); G(interp_quat);
 #undef G
   if (e == hipSuccess) e = hipMemcpy(&H.time, D.wtime, sizeof(double), hipMemcpyDeviceToHost);
   int counters[C_COUNT];
@@ -1639,6 +1684,35 @@ template <typename R> static int set_time_t(chp_ctx* ctx, uint32_t id, double ti
   return CHP_OK;
 }
 
+// World.step(dt, timeSinceLastCalled, maxSubSteps) (world_class.dart:400-431)
+template <typename R>
+static int step_interp_t(chp_ctx* ctx, uint32_t id, double dt, double elapsed, int maxsub, int32_t* substeps) {
+  WorldSlot& S = ctx->worlds[id];
+  double acc = S.accumulator + elapsed;
+  int n = 0;
+  while (acc >= dt && n < maxsub) {   // the steps; internalStep leaves World.time alone
+    acc -= dt;
+    n++;
+  }
+  if (substeps) *substeps = n;
+  if (n > 0) {
+    dev_of<R>(S).time_frozen = 1;
+    ctx->worlds_dirty = true;
+    int r = run_steps<R>(ctx, &id, 1, dt, n, true, nullptr);
+    dev_of<R>(S).time_frozen = 0;
+    ctx->worlds_dirty = true;
+    if (r != CHP_OK) return r;
+  }
+  acc = std::fmod(acc, dt);   // Dart's % on a non-negative double
+  S.accumulator = acc;
+  int r = prepare<R>(ctx, &id, 1, dt);
+  if (r != CHP_OK) return r;
+  k_interp<R><<<(S.nb + 255) / 256, 256, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, (int)id, acc / dt, elapsed);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CHP_OK;
+}
+
 template <typename R> static int broadphase_t(chp_ctx* ctx, uint32_t id, int32_t* pi, int32_t* pj, uint32_t cap, uint32_t* np) {
   int r = prepare<R>(ctx, &id, 1, ctx->worlds[id].last_dt > 0 ? ctx->worlds[id].last_dt : 1.0 / 60);
   if (r != CHP_OK) return r;
@@ -1706,6 +1780,111 @@ static int capture_t(chp_ctx* ctx, uint32_t id, double dt, int32_t* pi, int32_t*
   return CHP_OK;
 }
 
+
+static int ensure_qbuf(chp_ctx* ctx, size_t bytes) {
+  if (ctx->d_qbuf_cap >= bytes) return CHP_OK;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->d_qbuf) (void)hipFree(ctx->d_qbuf);
+  ctx->d_qbuf = nullptr;
+  ctx->d_qbuf_cap = 0;
+  HIPCHK(ctx, hipMalloc(&ctx->d_qbuf, bytes));
+  ctx->d_qbuf_cap = bytes;
+  return CHP_OK;
+}
+
+// the preparation every query starts with (aabbQuery's AABB updates and SAP sort)
+template <typename R> static int query_prep(chp_ctx* ctx, uint32_t id) {
+  int r = prepare<R>(ctx, &id, 1, ctx->worlds[id].last_dt > 0 ? ctx->worlds[id].last_dt : 1.0 / 60);
+  if (r != CHP_OK) return r;
+  k_query_prep<R><<<1, 1024, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, (int)id);
+  HIPCHK(ctx, hipGetLastError());
+  return CHP_OK;
+}
+
+template <typename R>
+static int aabb_query_t(chp_ctx* ctx, uint32_t id, const float* lo, const float* hi, int32_t* out, uint32_t cap,
+                        uint32_t* n) {
+  int r = query_prep<R>(ctx, id);
+  if (r != CHP_OK) return r;
+  const size_t ncap = std::max<uint32_t>(cap, 1);
+  r = ensure_qbuf(ctx, (ncap + 1) * sizeof(int));
+  if (r != CHP_OK) return r;
+  int* d = (int*)ctx->d_qbuf;
+  k_aabb_query<R><<<1, 1024, 0, ctx->stream>>>((const DevWorld<R>*)ctx->d_worlds, (int)id, mk3(lo[0], lo[1], lo[2]),
+                                              mk3(hi[0], hi[1], hi[2]), d + 1, (int)cap, d);
+  HIPCHK(ctx, hipGetLastError());
+  int total = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&total, d, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *n = (uint32_t)total;
+  if ((uint32_t)total > cap) return fail(ctx, CHP_E_CAPACITY, "body buffer too small: need " + std::to_string(total));
+  if (total) HIPCHK(ctx, hipMemcpy(out, d + 1, (size_t)total * sizeof(int), hipMemcpyDeviceToHost));
+  return CHP_OK;
+}
+
+template <typename R>
+static int raycast_t(chp_ctx* ctx, uint32_t id, const chp_ray* rays, uint32_t nr, chp_ray_hit* hits, uint32_t cap,
+                     uint32_t* n) {
+  WorldSlot& S = ctx->worlds[id];
+  int r = query_prep<R>(ctx, id);
+  if (r != CHP_OK) return r;
+  *n = 0;
+  if (nr == 0) return CHP_OK;
+  // staging: rays | probe (int4) | counts | offsets | hits
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t o_rays = 0, o_probe = up(nr * sizeof(chp_ray)), o_cnt = up(o_probe + nr * sizeof(int4)),
+               o_off = up(o_cnt + nr * sizeof(int)), o_hits = up(o_off + nr * sizeof(int));
+  const size_t hcap = std::max<uint32_t>(cap, 1);
+  r = ensure_qbuf(ctx, o_hits + hcap * sizeof(chp_ray_hit));
+  if (r != CHP_OK) return r;
+  char* q = (char*)ctx->d_qbuf;
+  chp_ray* d_rays = (chp_ray*)(q + o_rays);
+  int4* d_probe = (int4*)(q + o_probe);
+  int* d_cnt = (int*)(q + o_cnt);
+  int* d_off = (int*)(q + o_off);
+  chp_ray_hit* d_hits = (chp_ray_hit*)(q + o_hits);
+  hipStream_t st = ctx->stream;
+  const DevWorld<R>* dw = (const DevWorld<R>*)ctx->d_worlds;
+  DevWorld<R>& D = dev_of<R>(S);
+  HIPCHK(ctx, hipMemcpyAsync(d_rays, rays, nr * sizeof(chp_ray), hipMemcpyHostToDevice, st));
+  const int blocks = (int)std::min<uint32_t>((nr + 63) / 64, 4096);
+  // the world's ray has no heightfield cell index yet: the first ray (in order) whose walk calls
+  // the heightfield intersection sets it
+  if (!((HostWorld<R>*)S.host.get())->hf_nx.empty()) {
+    int rs[4];
+    HIPCHK(ctx, hipMemcpyAsync(rs, D.ray_state, sizeof rs, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    if (!rs[RS_SET]) {
+      k_rays<R, 0><<<blocks, 64, 0, st>>>(dw, (int)id, d_rays, (int)nr, d_probe, nullptr, nullptr, nullptr);
+      HIPCHK(ctx, hipGetLastError());
+      std::vector<int4> pr(nr);
+      HIPCHK(ctx, hipMemcpyAsync(pr.data(), d_probe, nr * sizeof(int4), hipMemcpyDeviceToHost, st));
+      HIPCHK(ctx, hipStreamSynchronize(st));
+      for (uint32_t k = 0; k < nr; k++)
+        if (pr[k].x) {
+          rs[RS_SET] = 1; rs[RS_X] = pr[k].y; rs[RS_Y] = pr[k].z;
+          HIPCHK(ctx, hipMemcpy(D.ray_state, rs, sizeof rs, hipMemcpyHostToDevice));
+          break;
+        }
+    }
+  }
+  k_rays<R, 1><<<blocks, 64, 0, st>>>(dw, (int)id, d_rays, (int)nr, nullptr, d_cnt, nullptr, nullptr);
+  HIPCHK(ctx, hipGetLastError());
+  std::vector<int> cnt(nr), off(nr);
+  HIPCHK(ctx, hipMemcpyAsync(cnt.data(), d_cnt, nr * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  size_t total = 0;
+  for (uint32_t k = 0; k < nr; k++) { off[k] = (int)total; total += (size_t)cnt[k]; }
+  *n = (uint32_t)std::min<size_t>(total, UINT32_MAX);
+  if (total > cap) return fail(ctx, CHP_E_CAPACITY, "hit buffer too small: need " + std::to_string(total));
+  if (total == 0) return CHP_OK;
+  HIPCHK(ctx, hipMemcpyAsync(d_off, off.data(), nr * sizeof(int), hipMemcpyHostToDevice, st));
+  k_rays<R, 2><<<blocks, 64, 0, st>>>(dw, (int)id, d_rays, (int)nr, nullptr, nullptr, d_off, d_hits);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(hits, d_hits, total * sizeof(chp_ray_hit), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  return CHP_OK;
+}
 
 static int ensure_eq(chp_ctx* ctx, size_t n) {
   if (ctx->d_eq_cap >= n) return CHP_OK;
@@ -2032,6 +2211,7 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   if (ctx->d_status) (void)hipFree(ctx->d_status);
   if (ctx->d_cptrs) (void)hipFree(ctx->d_cptrs);
   if (ctx->d_eq) (void)hipFree(ctx->d_eq);
+  if (ctx->d_qbuf) (void)hipFree(ctx->d_qbuf);
   if (ctx->d_q) (void)hipFree(ctx->d_q);
   if (ctx->d_qcls) (void)hipFree(ctx->d_qcls);
   for (hipEvent_t e : ctx->sev) (void)hipEventDestroy(e);
@@ -2233,6 +2413,60 @@ int chp_world_events(chp_ctx* ctx, uint32_t id, chp_event* out, uint32_t cap, ui
   }
   ctl[EV_N] = ctl[EV_NEED] = 0;
   HIPCHK(ctx, hipMemcpy(m + L.ctl, ctl, sizeof ctl, hipMemcpyHostToDevice));
+  return CHP_OK;
+}
+
+int chp_aabb_query(chp_ctx* ctx, uint32_t id, const float lower[3], const float upper[3], int32_t* bodies,
+                   uint32_t cap, uint32_t* n) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id) || !lower || !upper || !n || (cap && !bodies)) return fail(ctx, CHP_E_INVALID, "bad argument");
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? aabb_query_t<double>(ctx, id, lower, upper, bodies, cap, n)
+                                             : aabb_query_t<float>(ctx, id, lower, upper, bodies, cap, n);
+}
+
+int chp_raycast(chp_ctx* ctx, uint32_t id, const chp_ray* rays, uint32_t nrays, chp_ray_hit* hits, uint32_t cap,
+                uint32_t* n) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id) || !n || (nrays && !rays) || (cap && !hits)) return fail(ctx, CHP_E_INVALID, "bad argument");
+  for (uint32_t k = 0; k < nrays; k++) {
+    const chp_ray& r = rays[k];
+    bool ok = r.mode >= CHP_RAY_CLOSEST && r.mode <= CHP_RAY_ALL;
+    for (int c = 0; c < 3; c++) ok = ok && std::isfinite(r.from[c]) && std::isfinite(r.to[c]);
+    if (!ok) return fail(ctx, CHP_E_INVALID, "ray " + std::to_string(k) + ": bad mode or non-finite end points");
+  }
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? raycast_t<double>(ctx, id, rays, nrays, hits, cap, n)
+                                             : raycast_t<float>(ctx, id, rays, nrays, hits, cap, n);
+}
+
+int chp_world_step_interpolated(chp_ctx* ctx, uint32_t id, double dt, double elapsed, int32_t max_substeps,
+                                int32_t* substeps) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id)) return fail(ctx, CHP_E_INVALID, "unknown world");
+  if (!(dt > 0) || !(elapsed >= 0) || !std::isfinite(elapsed) || max_substeps < 0)
+    return fail(ctx, CHP_E_INVALID, "dt must be > 0, the elapsed time finite and >= 0, max_substeps >= 0");
+  (void)hipSetDevice(ctx->device);
+  return ctx->precision == CHP_PRECISION_F64 ? step_interp_t<double>(ctx, id, dt, elapsed, max_substeps, substeps)
+                                             : step_interp_t<float>(ctx, id, dt, elapsed, max_substeps, substeps);
+}
+
+int chp_interpolated_state(chp_ctx* ctx, uint32_t id, float* position, float* quaternion, uint32_t n) {
+  if (!ctx) return CHP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (!valid_world(ctx, id) || !position || !quaternion) return fail(ctx, CHP_E_INVALID, "bad argument");
+  WorldSlot& S = ctx->worlds[id];
+  if (n < (uint32_t)S.nb) return fail(ctx, CHP_E_CAPACITY, "buffers smaller than the body count");
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const void *p, *q;
+  if (ctx->precision == CHP_PRECISION_F64) { p = dev_of<double>(S).interp_pos; q = dev_of<double>(S).interp_quat; }
+  else { p = dev_of<float>(S).interp_pos; q = dev_of<float>(S).interp_quat; }
+  HIPCHK(ctx, hipMemcpy(position, p, (size_t)S.nb * 12, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(quaternion, q, (size_t)S.nb * 16, hipMemcpyDeviceToHost));
   return CHP_OK;
 }
 
@@ -2449,7 +2683,7 @@ uint32_t chp_abi_struct_size(const char* name) {
   SZ(chp_world_desc); SZ(chp_material_desc); SZ(chp_contact_material_desc); SZ(chp_shape_desc); SZ(chp_hull_desc);
   SZ(chp_heightfield_desc); SZ(chp_body_shape); SZ(chp_body_desc); SZ(chp_constraint_desc); SZ(chp_scene);
   SZ(chp_body_state); SZ(chp_body_info); SZ(chp_row); SZ(chp_step_stats); SZ(chp_equation); SZ(chp_request_status);
-  SZ(chp_event);
+  SZ(chp_event); SZ(chp_ray); SZ(chp_ray_hit);
 #undef SZ
   return 0;
 }

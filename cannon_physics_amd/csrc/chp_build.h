@@ -93,6 +93,8 @@ template <typename R> struct HostWorld {
   std::vector<float> inertia;
   std::vector<int> type, flags, group, mask, material, shape_first, shape_count, order;
   // sleep: state (CHP_SLEEP_*), Body.timeLastSleepy (a Dart double in every mode), limits
+  // Body.previous / interpolated pose: the constructor's (Body ctor, rigid_body.dart:50-70)
+  std::vector<float> prev_pos, prev_quat, interp_pos, interp_quat;
   std::vector<int> sleep_state;
   std::vector<double> sleep_tls, sleep_time;
   std::vector<R> sleep_speed;
@@ -122,6 +124,7 @@ template <typename R> struct HostWorld {
   chp_world_desc wd;
   int sap_axis = 0;
   double time = 0;   // World.time (kept on the host when the slab is rebuilt)
+  std::vector<int> ray_state = std::vector<int>(4, 0);   // the world's ray state (kept likewise)
 };
 
 // Shape.calculateWorldAABB for one body-shape instance, in FP64 (box.dart:149-192,
@@ -397,6 +400,8 @@ template <typename R> struct Builder {
       int fl = (d.collision_response ? BF_RESPONSE : 0) | (d.is_trigger ? BF_TRIGGER : 0) | (b.dirty ? BF_AABB_DIRTY : 0) |
                (d.allow_sleep ? BF_ALLOW_SLEEP : 0);
       if (d.sleep_state < CHP_SLEEP_AWAKE || d.sleep_state > CHP_SLEEP_SLEEPING) fail("bad body sleep state");
+      for (int k = 0; k < 3; k++) { W.prev_pos.push_back(d.position[k]); W.interp_pos.push_back(d.position[k]); }
+      for (int k = 0; k < 4; k++) { W.prev_quat.push_back(d.quaternion[k]); W.interp_quat.push_back(d.quaternion[k]); }
       W.sleep_state.push_back(d.sleep_state);
       W.sleep_tls.push_back(0.0);   // World.addBody: timeLastSleepy = world.time (0 at construction)
       W.sleep_speed.push_back((R)d.sleep_speed_limit);

@@ -294,7 +294,9 @@ __device__ __forceinline__ unsigned int order_key(float f) {
 // re-sorted by aabb.lowerBound[axis]; ties keep the previous order (the insertion sort is
 // stable), so sorting (key, previous position) reproduces it exactly.  Then the sweep with
 // its `continue` on filtered pairs and `break` on checkBounds (:41-63).
-template <typename R> __device__ bool st_bp_sap(const DevWorld<R>& W, int* sh) {
+// SAPBroadphase.sortList's insertion sort (sap_broadphase.dart:168-189) on one workgroup: the
+// axis list stably re-sorted by aabb.lowerBound[axis] (a bitonic sort of (key, position) pairs)
+template <typename R> __device__ void sap_sort(const DevWorld<R>& W) {
   int n = W.nb, ax = W.sap_axis;
   int np2 = 1;
   while (np2 < n) np2 <<= 1;
@@ -322,6 +324,10 @@ template <typename R> __device__ bool st_bp_sap(const DevWorld<R>& W, int* sh) {
     }
   for (int k = threadIdx.x; k < n; k += blockDim.x) W.order[k] = W.tmp[(int)(key[k] & 0xffffffffu)];
   __syncthreads();
+}
+template <typename R> __device__ bool st_bp_sap(const DevWorld<R>& W, int* sh) {
+  int n = W.nb, ax = W.sap_axis;
+  sap_sort(W);
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     int bi = W.order[k], c = 0;
     double A2 = (double)axis_of(F3<R>(W.pos, bi), ax) + (double)W.brad[bi];
@@ -1648,6 +1654,45 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
   g.sync();
 }
 
+// ---- interpolation (World.step with timeSinceLastCalled, world_class.dart:421-428) ----------------
+// Quaternion.slerp (quaternion.dart:118-168)
+template <typename R> __device__ __forceinline__ f4 slerp4(f4 a, f4 b, R t) {
+  const R ax = a.x, ay = a.y, az = a.z, aw = a.w;
+  R bx = b.x, by = b.y, bz = b.z, bw = b.w;
+  R cosom = ax * bx + ay * by + az * bz + aw * bw;
+  if (cosom < (R)0) {
+    cosom = -cosom;
+    bx = -bx; by = -by; bz = -bz; bw = -bw;
+  }
+  R s0, s1;
+  if ((R)1 - cosom > (R)0.000001) {
+    const R omega = acos(cosom), sinom = sin(omega);
+    s0 = sin(((R)1 - t) * omega) / sinom;
+    s1 = sin(t * omega) / sinom;
+  } else {
+    s0 = (R)1 - t;
+    s1 = t;
+  }
+  f4 r;
+  r.x = st<R>(s0 * ax + s1 * bx); r.y = st<R>(s0 * ay + s1 * by);
+  r.z = st<R>(s0 * az + s1 * bz); r.w = st<R>(s0 * aw + s1 * bw);
+  return r;
+}
+// body i: interpolatedPosition = previousPosition.lerp(position, t), interpolatedQuaternion =
+// previousQuaternion.slerp(quaternion, t), then previousQuaternion.normalize()
+template <typename R> __device__ void interp_body(const DevWorld<R>& W, int i, R t) {
+  ((f3*)W.interp_pos)[i] = lerp3<R>(((const f3*)W.prev_pos)[i], F3<R>(W.pos, i), t);
+  f4 a = ((const f4*)W.prev_quat)[i];
+  ((f4*)W.interp_quat)[i] = slerp4<R>(a, ((const f4*)W.quat)[i], t);
+  const R x = a.x, y = a.y, z = a.z, w = a.w;
+  const R l = sqrt((x * x) + (y * y) + (z * z) + (w * w));   // vector_math Quaternion.normalize
+  if (l != (R)0) {
+    const R d = (R)1 / l;
+    a.x = st<R>(x * d); a.y = st<R>(y * d); a.z = st<R>(z * d); a.w = st<R>(w * d);
+    ((f4*)W.prev_quat)[i] = a;
+  }
+}
+
 // ---- sleep ------------------------------------------------------------------------------------
 // the wake-up marks of one contact side (world_class.dart:562-590): sleeping dynamic body a,
 // allowed to sleep, touched by awake non-static body b whose speed^2 (|v|^2 + |w|^2) is at least
@@ -1704,6 +1749,9 @@ template <typename R> __device__ __forceinline__ void sleep_tick(const DevWorld<
 // non-finite
 template <typename R> __device__ bool post_body(const DevWorld<R>& W, int i, R dt, int E, bool qn) {
   bool ok = true;
+  // Body.integrate saves the pose first, for every body (rigid_body.dart:628-630)
+  ((f3*)W.prev_pos)[i] = F3<R>(W.pos, i);
+  ((f4*)W.prev_quat)[i] = ((const f4*)W.quat)[i];
   f3 v = F3<R>(W.vel, i), w = F3<R>(W.avel, i);
   f3 lf = F3<R>(W.lin_factor, i), af = F3<R>(W.ang_factor, i);
   if (E != 0) {
@@ -1774,7 +1822,7 @@ template <typename R> __device__ __forceinline__ bool post_qn(const DevWorld<R>&
 // the end of a step: the step counter, World.time += dt (world_class.dart:397-399)
 template <typename R> __device__ __forceinline__ void step_done(const DevWorld<R>& W, double dt) {
   W.counters[C_STEP] += 1;
-  W.wtime[0] += dt;
+  if (!W.time_frozen) W.wtime[0] += dt;
 }
 template <typename R> __device__ void st_post(const DevWorld<R>& W, double dt) {
   const int E = W.counters[C_ROWS];

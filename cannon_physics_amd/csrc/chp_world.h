@@ -75,6 +75,7 @@ template <typename R> struct DevWorld {
   // parameters
   int broadphase, sap_axis, iterations, quat_skip;
   int quat_fast, has_filtered_constraints, stepnumber, allow_sleep;   // allow_sleep: World.allowSleep
+  int time_frozen, pad_tf;   // 1: the steps do not advance World.time (internalStep under step(dt, t))
   float gravity[3];
   float pad1;
   R fric_grav_len, tol2;
@@ -84,6 +85,9 @@ template <typename R> struct DevWorld {
   CHP_G(R) mass, inv_mass, brad, ldamp, adamp;
   CHP_G(int) type, flags, group, mask, material, shape_first, shape_count;
   CHP_G(int) order;  // SAPBroadphase.axisList (body indices), persistent across steps
+  // Body.previousPosition / previousQuaternion (saved by every step's integration) and
+  // interpolatedPosition / interpolatedQuaternion (World.step with timeSinceLastCalled)
+  CHP_G(float) prev_pos, prev_quat, interp_pos, interp_quat;
   // sleep (rigid_body.dart:263-314): state per body, Body.timeLastSleepy, the limits; the
   // world's time (World.time, the sleep timer's clock) in wtime[0]
   CHP_G(int) sleep_state;
@@ -133,6 +137,7 @@ template <typename R> struct DevWorld {
   CHP_G(int) sgm;
   CHP_G(R) sbr;
   CHP_G(int) counters;   // [0] status [1] pairs [2] contacts [3] rows [4] iterations [5] levels [6] staging
+  CHP_G(int) ray_state;  // the world's ray: the heightfield cell index its first call kept
   // contact events (null / 0 when off): the overlap keepers' sorted keys (body and shape sets,
   // two of each: the previous step's and the current one, by parity), two sort buffers of kcap
   // keys, scan scratch, the justTest overlaps of the step (bi, bj, si, sj) and the event buffer

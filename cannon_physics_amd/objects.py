@@ -202,7 +202,50 @@ class Heightfield(Shape):
         self.elementSize = float(elementSize)
 
 
-class Body:
+class EventTarget:
+    """EventTarget (lib/utils/event_target.dart): listeners per event type, called in the order
+    they were added."""
+
+    def addEventListener(self, type, listener):
+        ls = self.__dict__.setdefault("_listeners", {}).setdefault(type, [])
+        if listener not in ls:
+            ls.append(listener)
+        self._listeners_changed()
+        return self
+
+    def hasEventListener(self, type, listener):
+        return listener in self.__dict__.get("_listeners", {}).get(type, [])
+
+    def hasAnyEventListener(self, type):
+        return bool(self.__dict__.get("_listeners", {}).get(type))
+
+    def removeEventListener(self, type, listener):
+        ls = self.__dict__.get("_listeners", {}).get(type, [])
+        if listener in ls:
+            ls.remove(listener)
+        self._listeners_changed()
+        return self
+
+    def dispatchEvent(self, event):
+        event.target = self
+        for fn in list(self.__dict__.get("_listeners", {}).get(event.type, [])):
+            fn(event)
+        return self
+
+    def _listeners_changed(self):
+        pass
+
+
+class Event:
+    """An event object: type, target and the fields the dispatcher sets."""
+
+    def __init__(self, type, **fields):
+        self.type = type
+        self.target = None
+        self.__dict__.update(fields)
+
+
+class Body(EventTarget):
     """Body({...}) with the reference defaults (rigid_body.dart:27-86)."""
 
     def __init__(self, mass=0.0, position=None, velocity=None, quaternion=None,
@@ -240,6 +283,10 @@ class Body:
         self.index = -1
         if shape is not None:
             self.addShape(shape)
+
+    def _listeners_changed(self):
+        if self.world is not None:
+            self.world._listeners_changed()
 
     def sleep(self):
         """Body.sleep() (rigid_body.dart:272-278), before the world's first step."""

@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _abi as A
 from ._lib import check, load
-from .objects import (GSSolver, NaiveBroadphase, Vector3, _DefaultContactMaterial, _v3,
+from .objects import (Event, EventTarget, GSSolver, NaiveBroadphase, Vector3, _DefaultContactMaterial, _v3,
                       compile_scene)
 
 
@@ -138,6 +138,45 @@ class Context:
         self.check(self.lib.chp_world_events(self.ptr, wid, buf, n.value, C.byref(n)))
         return np.array(np.ctypeslib.as_array(buf)[: n.value])
 
+    def aabb_query(self, wid, lower, upper):
+        """Broadphase.aabbQuery: body indices in the broadphase's order"""
+        lo, hi = (C.c_float * 3)(*lower), (C.c_float * 3)(*upper)
+        n = C.c_uint32()
+        cap = max(1, self.num_bodies(wid))
+        out = (C.c_int32 * cap)()
+        self.check(self.lib.chp_aabb_query(self.ptr, wid, lo, hi, out, cap, C.byref(n)))
+        return np.array(out[: n.value], dtype=np.int32)
+
+    def raycast(self, wid, rays, cap=None):
+        """World.raycast* for an array of A.Ray: structured array of A.RayHit, grouped by ray"""
+        nr = len(rays)
+        arr = (A.Ray * max(1, nr))(*rays)
+        cap = cap if cap is not None else max(16, 4 * nr)
+        while True:
+            out = (A.RayHit * cap)()
+            n = C.c_uint32()
+            rc = self.lib.chp_raycast(self.ptr, wid, arr, nr, out, cap, C.byref(n))
+            if rc == A.CHP_E_CAPACITY and n.value > cap:
+                cap = n.value
+                continue
+            self.check(rc)
+            return np.array(np.ctypeslib.as_array(out)[: n.value])
+
+    def step_interpolated(self, wid, dt, elapsed, max_substeps=10):
+        """World.step(dt, timeSinceLastCalled, maxSubSteps): returns the substeps taken"""
+        n = C.c_int32()
+        self.check(self.lib.chp_world_step_interpolated(self.ptr, wid, dt, elapsed, max_substeps, C.byref(n)))
+        return int(n.value)
+
+    def interpolated(self, wid):
+        """(interpolatedPosition (n, 3), interpolatedQuaternion (n, 4)) as float32 arrays"""
+        n = self.num_bodies(wid)
+        p = np.zeros((max(1, n), 3), dtype=np.float32)
+        q = np.zeros((max(1, n), 4), dtype=np.float32)
+        self.check(self.lib.chp_interpolated_state(self.ptr, wid, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                                   q.ctypes.data_as(C.POINTER(C.c_float)), max(1, n)))
+        return p[:n], q[:n]
+
     def set_time(self, wid, t):
         """World.time of a world (the clock of the sleep timer)"""
         self.check(self.lib.chp_world_set_time(self.ptr, wid, float(t)))
@@ -242,7 +281,7 @@ class _BodyView:
     """Reads of body state after the world went to the device."""
 
 
-class World:
+class World(EventTarget):
     """World({gravity, frictionGravity, allowSleep, broadphase, solver, quatNormalizeFast,
     quatNormalizeSkip}) (world_class.dart:135-162).  With allowSleep the device runs the sleep
     timer of every body after each step (world_class.dart:686-699)."""
@@ -268,6 +307,8 @@ class World:
         self._precision = precision
         self._wid = None
         self._state = None
+        self._events_on = False
+        self._shapes = None
 
     # ---- construction (mirrors the reference) ----
     def addBody(self, body):
@@ -311,19 +352,96 @@ class World:
         return self._ensure_device()
 
     def step(self, dt, timeSinceLastCalled=None, maxSubSteps=10):
-        """Fixed stepping (world_class.dart:392-399)."""
-        if timeSinceLastCalled is not None:
-            raise NotImplementedError("interpolated stepping is not part of the device pipeline")
-        self.step_n(dt, 1)
-
-    def step_n(self, dt, n):
-        """n fixed steps; World.time is the device's sum of the steps' dt."""
+        """World.step (world_class.dart:391-431): fixed stepping, or with timeSinceLastCalled the
+        accumulator's substeps and the bodies' interpolated poses (see interpolated())."""
+        if timeSinceLastCalled is None:
+            self.step_n(dt, 1)
+            return
         wid = self._ensure_device()
         self.dt = dt
+        self._sync_events()
+        n = self.context.step_interpolated(wid, dt, timeSinceLastCalled, maxSubSteps)
+        self.time = self.context.stats(wid)["time"]
+        self.stepnumber += n
+        self._state = None
+        if self._events_on:
+            self._dispatch(self.context.events(wid))
+
+    def interpolated(self):
+        """Body.interpolatedPosition / interpolatedQuaternion of every body (float32 arrays)"""
+        return self.context.interpolated(self._ensure_device())
+
+    def step_n(self, dt, n):
+        """n fixed steps; World.time is the device's sum of the steps' dt.  Contact events
+        (collide on the bodies; begin/endContact, begin/endShapeContact on the world) are
+        dispatched after the steps, in the order the steps raised them."""
+        wid = self._ensure_device()
+        self.dt = dt
+        self._sync_events()
         self.context.step([wid], dt, n)
         self.time = self.context.stats(wid)["time"]
         self.stepnumber += n
         self._state = None
+        if self._events_on:
+            self._dispatch(self.context.events(wid))
+
+    # ---- contact events (world_class.dart:543-610, 703-770) ----
+    _EVENT_TYPES = ("beginContact", "endContact", "beginShapeContact", "endShapeContact")
+
+    def _wants_events(self):
+        return any(self.hasAnyEventListener(t) for t in self._EVENT_TYPES) or any(
+            b.hasAnyEventListener("collide") for b in self.bodies)
+
+    def _listeners_changed(self):
+        if self._wid is not None:
+            self._sync_events()
+
+    def _sync_events(self):
+        want = self._wants_events()
+        if want != self._events_on:
+            if self._wid is not None:
+                self.context.set_events(self._wid, 0x1F if want else 0)
+                self._events_on = want
+            elif not want:
+                self._events_on = False
+            else:   # switched on when the world goes to the device
+                self.context.set_events(self._ensure_device(), 0x1F)
+                self._events_on = True
+
+    def _dispatch(self, events):
+        table = self._shape_table()
+        for e in events:
+            kind = int(e["kind"])
+            if kind == A.EVENT_COLLIDE:
+                bi, bj = self.bodies[int(e["body_a"])], self.bodies[int(e["body_b"])]
+                contact = Event("contact", bi=bi, bj=bj, si=table[int(e["shape_a"])], sj=table[int(e["shape_b"])],
+                                ni=Vector3(*e["ni"].tolist()), ri=Vector3(*e["ri"].tolist()),
+                                rj=Vector3(*e["rj"].tolist()))
+                bi.dispatchEvent(Event("collide", body=bj, contact=contact))
+                bj.dispatchEvent(Event("collide", body=bi, contact=contact))
+                continue
+            name = A.EVENT_NAMES[kind]
+            if not self.hasAnyEventListener(name):
+                continue
+            ba = self.bodies[int(e["body_a"])] if e["body_a"] >= 0 else None
+            bb = self.bodies[int(e["body_b"])] if e["body_b"] >= 0 else None
+            if kind in (A.EVENT_BEGIN_CONTACT, A.EVENT_END_CONTACT):
+                self.dispatchEvent(Event(name, bodyA=ba, bodyB=bb))
+            else:
+                self.dispatchEvent(Event(name, bodyA=ba, bodyB=bb, shapeA=table[int(e["shape_a"])],
+                                         shapeB=table[int(e["shape_b"])]))
+
+    def _shape_table(self):
+        """scene shape index -> the mirrored Shape object (the recipe's shape order)"""
+        if getattr(self, "_shapes", None) is None:
+            seen, table = set(), []
+            for b in self.bodies:
+                for s in b.shapes:
+                    if id(s) not in seen:
+                        seen.add(id(s))
+                        table.append(s)
+            self._shapes = table
+        return self._shapes
 
     @property
     def hasActiveBodies(self):

@@ -350,6 +350,62 @@ int chp_world_set_events(chp_ctx* ctx, uint32_t world_id, uint32_t kind_mask, ui
  * the capacity chp_world_set_events needs for that many steps). */
 int chp_world_events(chp_ctx* ctx, uint32_t world_id, chp_event* out, uint32_t cap, uint32_t* n);
 
+/* ---- queries between steps --------------------------------------------------------------- */
+/* Broadphase.aabbQuery (naive_broadphase.dart:39-56, sap_broadphase.dart:239-261): the bodies
+ * whose AABB overlaps [lower, upper] (AABB.overlaps, aabb.dart:131-147), in the broadphase's
+ * order (body order; SAP: the axis list, sorted first).  As in the reference, every body whose
+ * AABB is out of date gets it updated (and SAP's axis list stays sorted). */
+int chp_aabb_query(chp_ctx* ctx, uint32_t world_id, const float lower[3], const float upper[3], int32_t* bodies,
+                   uint32_t cap, uint32_t* n);
+
+/* World.raycastClosest / raycastAny / raycastAll (world_class.dart:245-276) through
+ * Ray.intersectWorld (ray_class.dart:176-204): each ray against the bodies aabbQuery returns
+ * for its AABB, spheres, planes, boxes, convex hulls and heightfields. */
+enum { CHP_RAY_CLOSEST = 0, CHP_RAY_ANY = 1, CHP_RAY_ALL = 2 };   /* RayMode */
+typedef struct chp_ray {
+  float from[3], to[3];
+  int32_t mode;                          /* CHP_RAY_* */
+  int32_t skip_backfaces;                /* RayOptions.skipBackfaces (the World's default: 1) */
+  int32_t check_collision_response;      /* RayOptions.checkCollisionResponse (default 1) */
+  int32_t collision_filter_group;        /* default -1 */
+  int32_t collision_filter_mask;         /* default -1 */
+  int32_t reserved;
+} chp_ray;
+typedef struct chp_ray_hit {             /* RaycastResult */
+  int32_t ray;                           /* the ray's index */
+  int32_t body, shape;                   /* world-local body, scene shape */
+  int32_t face;                          /* hitFaceIndex: the face of the last intersection the
+                                            ray reported (closest: not always the kept one, as in
+                                            the reference); -1 for spheres and planes */
+  float normal[3], point[3];             /* hitNormalWorld, hitPointWorld */
+  double distance;
+} chp_ray_hit;
+/* The hits of the rays, grouped by ray in ray order: closest and any give at most one each,
+ * all gives every hit in the order of its callbacks.  CHP_E_CAPACITY when cap is too small
+ * (*n = the number of hits).  A world's ray keeps state between calls, as the reference's
+ * does: its first heightfield intersection fixes the cell window of every later one
+ * (ray_class.dart:364-371 reads entries 0 and 1 of an index list that only grows). */
+int chp_raycast(chp_ctx* ctx, uint32_t world_id, const chp_ray* rays, uint32_t nrays, chp_ray_hit* hits,
+                uint32_t cap, uint32_t* n);
+
+/* ---- stepping with interpolation --------------------------------------------------------- */
+/* World.step(dt, timeSinceLastCalled, maxSubSteps) (world_class.dart:391-431), the stepping
+ * World.fixedStep drives (:361-379): the world's accumulator (starting at 0) grows by
+ * time_since_last_called; fixed steps of dt run while it holds one, at most max_substeps
+ * (internalStep: World.time does not advance per step); the accumulator keeps the remainder
+ * (accumulator % dt); every body's interpolated pose is its pose before the last step
+ * (Body.previousPosition / previousQuaternion, saved by every step) moved towards the current
+ * one by accumulator / dt (lerp / slerp), and World.time += time_since_last_called.  The
+ * reference also stops taking steps once the wall clock has run past dt; this call does not,
+ * so its steps are the same on every machine.  *substeps (may be NULL) receives the steps
+ * taken. */
+int chp_world_step_interpolated(chp_ctx* ctx, uint32_t world_id, double dt, double time_since_last_called,
+                                int32_t max_substeps, int32_t* substeps);
+/* Body.interpolatedPosition / interpolatedQuaternion of every body after the last
+ * chp_world_step_interpolated (before the first: the pose each body was constructed with);
+ * position: 3 floats per body, quaternion: 4 (x, y, z, w). */
+int chp_interpolated_state(chp_ctx* ctx, uint32_t world_id, float* position, float* quaternion, uint32_t n);
+
 /* ---- slot-level entry points (one world, current device state) --------------------------- */
 /* Broadphase.collisionPairs: pairs in reference emission order (world-local body indices). */
 int chp_broadphase(chp_ctx* ctx, uint32_t world_id, int32_t* pi, int32_t* pj, uint32_t cap,

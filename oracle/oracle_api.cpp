@@ -81,6 +81,62 @@ void orc_step(void* wp, double dt, int n) {
   }
 }
 double orc_time(void* wp) { return ((World*)wp)->time; }
+
+/* Quaternion.slerp, quaternion.dart:118-168 */
+static void slerp(const orc::Q4& a, const orc::Q4& b, R t, orc::Q4& target) {
+  R ax = a.x, ay = a.y, az = a.z, aw = a.w;
+  R bx = b.x, by = b.y, bz = b.z, bw = b.w;
+  R cosom = ax * bx + ay * by + az * bz + aw * bw;
+  if (cosom < (R)0) { cosom = -cosom; bx = -bx; by = -by; bz = -bz; bw = -bw; }
+  R scale0, scale1;
+  if ((R)1 - cosom > (R)0.000001) {
+    R omega = std::acos(cosom), sinom = std::sin(omega);
+    scale0 = std::sin(((R)1 - t) * omega) / sinom;
+    scale1 = std::sin(t * omega) / sinom;
+  } else {
+    scale0 = (R)1 - t;
+    scale1 = t;
+  }
+  target.x = orc::S(scale0 * ax + scale1 * bx); target.y = orc::S(scale0 * ay + scale1 * by);
+  target.z = orc::S(scale0 * az + scale1 * bz); target.w = orc::S(scale0 * aw + scale1 * bw);
+}
+/* World.step(dt, timeSinceLastCalled, maxSubSteps), world_class.dart:400-431, without the
+ * wall-clock bail-out; returns the substeps taken */
+int orc_step_interpolated(void* wp, double dt, double elapsed, int maxSubSteps) {
+  World& w = *(World*)wp;
+  w.accumulator += elapsed;
+  int substeps = 0;
+  while (w.accumulator >= dt && substeps < maxSubSteps) {
+    internal_step(w, (R)dt);
+    w.accumulator -= dt;
+    substeps++;
+  }
+  w.accumulator = std::fmod(w.accumulator, dt);
+  double t = w.accumulator / dt;
+  for (Body* b : w.bodies) {
+    lerp(b->previousPosition, b->position, (R)t, b->interpolatedPosition);
+    slerp(b->previousQuaternion, b->quaternion, (R)t, b->interpolatedQuaternion);
+    orc::Q4& q = b->previousQuaternion;   /* previousQuaternion.normalize() */
+    R x = q.x, y = q.y, z = q.z, ww = q.w;
+    R l = std::sqrt((x * x) + (y * y) + (z * z) + (ww * ww));
+    if (l != (R)0) {
+      R d = (R)1 / l;
+      q.x = orc::S(x * d); q.y = orc::S(y * d); q.z = orc::S(z * d); q.w = orc::S(ww * d);
+    }
+  }
+  w.time += elapsed;
+  return substeps;
+}
+void orc_interpolated(void* wp, float* position, float* quaternion) {
+  World& w = *(World*)wp;
+  for (size_t i = 0; i < w.bodies.size(); i++) {
+    const Body& b = *w.bodies[i];
+    position[3 * i] = b.interpolatedPosition.x; position[3 * i + 1] = b.interpolatedPosition.y;
+    position[3 * i + 2] = b.interpolatedPosition.z;
+    quaternion[4 * i] = b.interpolatedQuaternion.x; quaternion[4 * i + 1] = b.interpolatedQuaternion.y;
+    quaternion[4 * i + 2] = b.interpolatedQuaternion.z; quaternion[4 * i + 3] = b.interpolatedQuaternion.w;
+  }
+}
 /* contact events: keepers and the event list run from the next step on */
 void orc_set_events(void* wp, int on) { ((World*)wp)->eventsOn = on != 0; }
 /* moves the recorded events out when cap holds them all; returns how many there are */

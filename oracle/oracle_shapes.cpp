@@ -475,6 +475,8 @@ World* build_world(const chp_scene* sc) {
     b->type = b->mass <= (R)0 ? CHP_BODY_STATIC : CHP_BODY_DYNAMIC;
     if (d.type >= 0) b->type = d.type;
     b->quaternion = f4(d.quaternion);
+    b->previousPosition = b->interpolatedPosition = b->position;   /* rigid_body.dart:50-70 */
+    b->previousQuaternion = b->interpolatedQuaternion = b->quaternion;
     b->angularVelocity = f3(d.angular_velocity);
     b->linearFactor = f3(d.linear_factor);
     b->angularFactor = f3(d.angular_factor);

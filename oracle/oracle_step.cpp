@@ -42,6 +42,24 @@ static void boundingSphereTest(Body* a, Body* b, std::vector<Body*>& p1, std::ve
 
 static float axisOf(const V3& v, int axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
 
+/* SAPBroadphase.sortList, sap_broadphase.dart:168-189 */
+void sap_sort_list(World& w) {
+  std::vector<Body*>& a = w.axisList;
+  int N = (int)a.size();
+  int axis = w.sapAxis;
+  for (int i = 0; i != N; i++)
+    if (a[i]->aabbNeedsUpdate) body_update_aabb(*a[i]);
+  for (int i = 1; i < N; i++) {  /* insertionSort{X,Y,Z}, :66-113 */
+    Body* v = a[i];
+    int j;
+    for (j = i - 1; j >= 0; j--) {
+      if ((double)axisOf(a[j]->aabbLower, axis) <= (double)axisOf(v->aabbLower, axis)) break;
+      a[j + 1] = a[j];
+    }
+    a[j + 1] = v;
+  }
+}
+
 void broadphase_pairs(World& w, std::vector<Body*>& p1, std::vector<Body*>& p2) {
   if (w.broadphase == CHP_BROADPHASE_NAIVE) {  /* naive_broadphase.dart:14-33 */
     int n = (int)w.bodies.size();
@@ -59,17 +77,7 @@ void broadphase_pairs(World& w, std::vector<Body*>& p1, std::vector<Body*>& p2) 
   int N = (int)a.size();
   int axis = w.sapAxis;
   if (w.sapDirty) {
-    for (int i = 0; i != N; i++)
-      if (a[i]->aabbNeedsUpdate) body_update_aabb(*a[i]);
-    for (int i = 1; i < N; i++) {  /* insertionSort{X,Y,Z}, :66-113 */
-      Body* v = a[i];
-      int j;
-      for (j = i - 1; j >= 0; j--) {
-        if ((double)axisOf(a[j]->aabbLower, axis) <= (double)axisOf(v->aabbLower, axis)) break;
-        a[j + 1] = a[j];
-      }
-      a[j + 1] = v;
-    }
+    sap_sort_list(w);
     w.sapDirty = false;
   }
   for (int i = 0; i != N; i++) {
@@ -875,6 +883,10 @@ struct NP {
   }
 };
 
+std::pair<std::shared_ptr<Hull>, V3> hf_pillar(Heightfield& hf, int xi, int yi, bool upper) {
+  return NP::pillar(hf, xi, yi, upper);
+}
+
 void hullComputeNormalsExt(Hull& h);
 void hullComputeEdgesExt(Hull& h);
 void hullBsrExt(Hull& h);
@@ -1124,6 +1136,8 @@ static void body_sleep_tick(Body& b, double time) {
 }
 
 static void body_integrate(Body& b, R dt, bool quatNormalize, bool fast) {
+  b.previousPosition = b.position;   /* rigid_body.dart:628-630 */
+  b.previousQuaternion = b.quaternion;
   if (!(b.type == CHP_BODY_DYNAMIC || b.type == CHP_BODY_KINEMATIC) || b.sleepState == CHP_SLEEP_SLEEPING) return;
   R iMdt = b.invMass * dt;
   b.velocity.x = S(L(b.velocity.x) + L(b.force.x) * iMdt * L(b.linearFactor.x));

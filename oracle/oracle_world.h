@@ -77,6 +77,9 @@ struct Body {
   double sleepTimeLimit = 1;
   double timeLastSleepy = 0;
   bool wakeUpAfterNarrowphase = false;
+  /* rigid_body.dart:132-190: the pose before the last integration and the interpolated one */
+  V3 previousPosition, interpolatedPosition;
+  Q4 previousQuaternion, interpolatedQuaternion;
 };
 
 struct Jac { V3 spatial, rotational; };
@@ -181,6 +184,7 @@ struct World {
   int stepnumber = 0;
   bool allowSleep = false;   /* World.allowSleep */
   double time = 0;           /* World.time */
+  double accumulator = 0;    /* World.accumulator */
   ContactMat defaultCM;
   std::vector<MaterialDesc> materials;
   std::map<std::pair<int, int>, ContactMat> cmTable;
@@ -195,6 +199,9 @@ struct World {
   OverlapKeeper bodyOverlapKeeper, shapeOverlapKeeper;
   std::vector<chp_event> events;
   std::vector<int> shapeBody;   /* Shape.body per scene shape */
+  /* World._tmpRay's heightfield index list (ray_class.dart:150): only its first two entries
+   * are ever read */
+  std::vector<int> rayHfIndex;
   /* per-step scratch */
   std::vector<Equation> contacts;
   std::vector<Equation> frictions;
@@ -205,5 +212,7 @@ World* build_world(const chp_scene* scene);
 void internal_step(World& w, R dt);
 void broadphase_pairs(World& w, std::vector<Body*>& p1, std::vector<Body*>& p2);
 void body_update_aabb(Body& b);
+void sap_sort_list(World& w);
+std::pair<std::shared_ptr<Hull>, V3> hf_pillar(Heightfield& hf, int xi, int yi, bool upper);
 
 }  // namespace orc

@@ -48,6 +48,10 @@ def lib(mode="f64"):
         "orc_sat": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_double)]),
         "orc_time": (C.c_double, [vp]),
         "orc_set_events": (None, [vp, C.c_int]),
+        "orc_step_interpolated": (C.c_int, [vp, C.c_double, C.c_double, C.c_int]),
+        "orc_interpolated": (None, [vp, P(C.c_float), P(C.c_float)]),
+        "orc_aabb_query": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_int), C.c_int]),
+        "orc_raycast": (C.c_int, [vp, P(A.Ray), C.c_int, P(A.RayHit), C.c_int]),
         "orc_events": (C.c_int, [vp, P(A.Event), C.c_int]),
         "orc_set_time": (None, [vp, C.c_double]),
     }
@@ -127,6 +131,31 @@ class OracleWorld:
 
     def contacts(self):
         return self.L.orc_capture_contacts(self.ptr)
+
+    def aabb_query(self, lower, upper):
+        lo, hi = (C.c_float * 3)(*lower), (C.c_float * 3)(*upper)
+        out = (C.c_int * max(1, self.n))()
+        n = self.L.orc_aabb_query(self.ptr, lo, hi, out, self.n)
+        return np.array(out[:n], dtype=np.int32)
+
+    def raycast(self, rays):
+        """World.raycast* for an array of A.Ray (the world's ray keeps its state)"""
+        nr = len(rays)
+        arr = (A.Ray * max(1, nr))(*rays)
+        cap = max(256, 64 * nr)
+        out = (A.RayHit * cap)()
+        n = self.L.orc_raycast(self.ptr, arr, nr, out, cap)
+        assert n <= cap, "raise the hit capacity"
+        return np.array(np.ctypeslib.as_array(out)[:n])
+
+    def step_interpolated(self, dt, elapsed, max_substeps=10):
+        return self.L.orc_step_interpolated(self.ptr, dt, elapsed, max_substeps)
+
+    def interpolated(self):
+        p = np.zeros((max(1, self.n), 3), dtype=np.float32)
+        q = np.zeros((max(1, self.n), 4), dtype=np.float32)
+        self.L.orc_interpolated(self.ptr, p.ctypes.data_as(C.POINTER(C.c_float)), q.ctypes.data_as(C.POINTER(C.c_float)))
+        return p[: self.n], q[: self.n]
 
     def set_events(self, on=True):
         """keep the overlap keepers and record the contact events from the next step on"""

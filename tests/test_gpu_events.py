@@ -144,3 +144,33 @@ def test_narrowphase_slot_names_contact_shapes():
         assert checked > 0
     finally:
         c.destroy_world(wid)
+
+
+def test_world_listeners_receive_the_device_events():
+    """World / Body listeners of the mirrored API get the device's events, in order"""
+    w = scenes.events_world(34)
+    rec = w.recipe()
+    got = []
+    w.addEventListener("beginContact", lambda e: got.append((A.EVENT_BEGIN_CONTACT, e.bodyA.index, e.bodyB.index)))
+    w.addEventListener("endShapeContact", lambda e: got.append((A.EVENT_END_SHAPE_CONTACT, e.bodyA.index, e.bodyB.index)))
+    probe = w.bodies[1]
+    probe.addEventListener("collide", lambda e: got.append((A.EVENT_COLLIDE, probe.index, e.body.index)))
+    ow = O.OracleWorld(rec, "f32")
+    ow.set_events(True)
+    for k in range(80):
+        if k % 2:
+            w.step(DT)
+            ow.step(DT, 1)
+        else:
+            w.step_n(DT, 2)
+            ow.step(DT, 2)
+        if k >= 40:
+            break
+    want = []
+    for e in ow.events():
+        kind, a, b = int(e["kind"]), int(e["body_a"]), int(e["body_b"])
+        if kind == A.EVENT_COLLIDE and probe.index in (a, b):
+            want.append((kind, probe.index, b if a == probe.index else a))
+        elif kind in (A.EVENT_BEGIN_CONTACT, A.EVENT_END_SHAPE_CONTACT):
+            want.append((kind, a, b))
+    assert got == want and len(want) > 0
