@@ -22,6 +22,7 @@ BROADPHASE_NAIVE, BROADPHASE_SAP = 0, 1
  CONSTRAINT_CONE_TWIST) = range(5)
 ROW_FRICTION, ROW_CONTACT, ROW_ROTATIONAL, ROW_CONE, ROW_MOTOR = range(5)
 PRECISION_F32, PRECISION_F64 = 0, 1
+SOLVER_GS, SOLVER_SPLIT = 0, 1
 SLEEP_AWAKE, SLEEP_SLEEPY, SLEEP_SLEEPING = 0, 1, 2
 
 F3 = C.c_float * 3
@@ -48,7 +49,7 @@ class WorldDesc(C.Structure):
         ("friction_stiffness", C.c_double),
         ("friction_relaxation", C.c_double),
         ("allow_sleep", C.c_int32),
-        ("reserved", C.c_int32),
+        ("solver", C.c_int32),
     ]
 
 

@@ -19,6 +19,7 @@
 #include "chp_stages.h"
 #include "chp_events.h"
 #include "chp_query.h"
+#include "chp_split.h"
 
 using namespace chp;
 
@@ -339,6 +340,7 @@ __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ 
   int id = group_world(worlds, list, nbig, n, sh, wave);
   if (id < 0) return;
   const DevWorld<R>& W = worlds[id];
+  if (W.solver_kind == CHP_SOLVER_SPLIT) return;   // k_split_prep orders its rows
   if (wave < 0) {
     if (W.nb <= big_cap) st_levels<R, GBlock, true>(W, GBlock(), sh, lds_i, lds_i + big_cap, LS_BIG, gather != 0);
     else st_levels<R, GBlock, false>(W, GBlock(), sh, W.tmp, nullptr, 0, gather != 0);
@@ -362,6 +364,7 @@ __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worl
   int id = group_world(worlds, list, nbig, n, sh, wave);
   if (id < 0) return;
   const DevWorld<R>& W = worlds[id];
+  if (W.solver_kind == CHP_SOLVER_SPLIT) return;   // k_split_gs solves its islands
   float* lds = (float*)lds_f4;
   const int L = W.counters[C_LEVELS];
   if (wave < 0) {
@@ -418,6 +421,24 @@ __global__ __launch_bounds__(64) void k_rays(const DevWorld<R>* __restrict__ wor
                                              int nr, int4* __restrict__ probe, int* __restrict__ cnt,
                                              const int* __restrict__ off, chp_ray_hit* __restrict__ out) {
   st_rays<R, PASS>(worlds[id], rays, nr, probe, cnt, off, out);
+}
+
+// SplitSolver: ids, islands and row positions (after the rows' body pairs), then the islands'
+// Gauss-Seidel (after the rows)
+template <typename R>
+__global__ __launch_bounds__(1024) void k_split_prep(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list) {
+  __shared__ int sh[1024];
+  CHP_WORLD_PROLOGUE
+  if (W.solver_kind != CHP_SOLVER_SPLIT) return;
+  st_split_prep(W, sh);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_split_gs(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                  int keep) {
+  __shared__ int sh[256];
+  CHP_WORLD_PROLOGUE
+  if (W.solver_kind != CHP_SOLVER_SPLIT) return;
+  st_split_gs(W, keep != 0);
 }
 
 // contact-event bookkeeping after the narrowphase (worlds with events on); seed: only the key
@@ -595,6 +616,7 @@ struct WorldSlot {
   size_t bytes = 0;
   double last_dt = -1;
   bool may_sleep = false;  // World.allowSleep, or a body asleep: the step runs the wake-up pass
+  bool split = false;      // World.solver is a SplitSolver
   double accumulator = 0;  // World.accumulator of World.step(dt, timeSinceLastCalled)
   // contact events: one device allocation (keys, sort buffers, scan, justTest list, control
   // words, events) while a kind mask is set
@@ -643,6 +665,7 @@ struct chp_ctx {
   int n_sap = 0;   // SAP worlds in the current list
   int n_sleep = 0; // worlds in the current list that may hold sleeping bodies
   int n_events = 0; // worlds in the current list with contact events on
+  int n_split = 0;  // worlds in the current list with a SplitSolver
   bool keep_rows = false;   // steps also write row records (the capture reads them)
   int* d_status = nullptr;
   const int** d_cptrs = nullptr;
@@ -665,7 +688,7 @@ struct chp_ctx {
     bool keep = false;
     double dt = 0;
     const void *dq = nullptr, *dqcls = nullptr, *dlist = nullptr, *dworlds = nullptr, *dbig = nullptr;
-    std::array<int, 7> big{};
+    std::array<int, 8> big{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } sg;
@@ -744,6 +767,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   D.has_filtered_constraints = H.has_filtered ? 1 : 0;
   D.allow_sleep = H.wd.allow_sleep ? 1 : 0;
   D.nshapes = H.nshapes;
+  D.solver_kind = H.wd.solver;
   for (int k = 0; k < 3; k++) D.gravity[k] = H.wd.gravity[k];
   {
     f3 g = H.wd.has_friction_gravity ? mk3(H.wd.friction_gravity[0], H.wd.friction_gravity[1], H.wd.friction_gravity[2])
@@ -795,7 +819,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   ARR(mat_friction, H.mat_friction); ARR(mat_restitution, H.mat_restitution); ARR(cm_table, H.cm_table);
   ARR(cm_params, H.cm_params);
   ARR(c_type, H.c_type); ARR(c_a, H.c_a); ARR(c_b, H.c_b); ARR(c_collide, H.c_collide); ARR(c_row_off, H.c_row_off);
-  ARR(c_eq_mask, H.c_eq_mask); ARR(c_vec, H.c_vec); ARR(c_eq, H.c_eq);
+  ARR(c_eq_mask, H.c_eq_mask); ARR(c_vec, H.c_vec); ARR(c_eq, H.c_eq); ARR(c_row_id, H.c_row_id);
   SCR(feff, float, 3 * nb); SCR(vl, float, 3 * nb); SCR(wl, float, 3 * nb); SCR(iiws, float, 9 * nb); SCR(ims, R, nb);
   int np2 = 1;
   while (np2 < nb) np2 <<= 1;
@@ -810,6 +834,24 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   SCR(vw, float, 8 * nb);
   SCR(last_level, int, std::max(nb, row_cap + 1));
   SCR(sort_keys, unsigned long long, np2 + nb);  // the grid-wide SAP sort merges between two halves
+  {  // SplitSolver: the persistent ids state (carried over from a previous slab) and the scratch
+    const bool split = H.wd.solver == CHP_SOLVER_SPLIT;
+    std::vector<int> ctl = split ? H.sp_ctl : std::vector<int>();
+    std::vector<int> cp = split ? H.sp_cpool : std::vector<int>(), fp = split ? H.sp_fpool : std::vector<int>();
+    if (split) {
+      ctl.resize(4, 0);
+      ctl[1] = std::min(ctl[1], contact_cap);   // pools beyond a smaller capacity are cut
+      ctl[2] = std::min(ctl[2], 2 * contact_cap);
+      cp.resize(contact_cap, 0);
+      fp.resize(2 * (size_t)contact_cap, 0);
+    }
+    ARR(sp_ctl, ctl); ARR(sp_cpool, cp); ARR(sp_fpool, fp);
+    const int cc = split ? contact_cap : 0, pc = split ? pair_cap : 0, rc = split ? row_cap : 0;
+    int rp2 = 1;
+    while (split && rp2 < rc) rp2 <<= 1;
+    SCR(sp_newc, int, cc); SCR(sp_newf, int, 2 * cc); SCR(sp_scan, int, 2 * pc + 2); SCR(row_id, int, rc);
+    SCR(sp_parent, int, split ? nb : 0); SCR(sp_keys, unsigned long long, split ? rp2 : 0); SCR(sp_vals, int, split ? rp2 : 0);
+  }
   const int nsr = (H.wd.broadphase == CHP_BROADPHASE_SAP && nb >= BP_BIG) ? nb : 0;
   SCR(sbp, float, 4 * nsr); SCR(sgm, int, 4 * nsr); SCR(sbr, R, nsr);
   size_t o_cnt = L.add<int>(C_COUNT);
@@ -838,6 +880,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
   S.last_dt = -1;
   S.may_sleep = H.wd.allow_sleep != 0;
   for (int v : H.sleep_state) S.may_sleep = S.may_sleep || v == CHP_SLEEP_SLEEPING;
+  S.split = H.wd.solver == CHP_SOLVER_SPLIT;
   S.shape_body = H.shape_body;
   S.devworld.assign((char*)&D, (char*)&D + sizeof(D));
   if (S.ev.mask) {   // a rebuilt slab keeps the event bookkeeping (sized for the new capacities)
@@ -1144,11 +1187,13 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   ctx->n_sap = 0;
   ctx->n_sleep = 0;
   ctx->n_events = 0;
+  ctx->n_split = 0;
   for (size_t li = 0; li < n; li++) {
     const WorldSlot& S = ctx->worlds[list[li]];
     if (S.broadphase == CHP_BROADPHASE_SAP) ctx->n_sap++;
     if (S.may_sleep) ctx->n_sleep++;
     if (S.ev.mask) ctx->n_events++;
+    if (S.split) ctx->n_split++;
     if (S.nb < BP_BIG) continue;
     bigw.push_back((int)li);
     for (int s0 = 0; s0 < S.nb; s0 += BP_SPAN) { spans.push_back((int)li); spans.push_back(s0); }
@@ -1190,7 +1235,7 @@ struct StepView {
   const void* worlds = nullptr;
   const int* list = nullptr;
   const int* big = nullptr;
-  int nspans = 0, ntiles = 0, nw = 0, rounds = 0, n_sap = 0, n_sleep = 0, n_events = 0;
+  int nspans = 0, ntiles = 0, nw = 0, rounds = 0, n_sap = 0, n_sleep = 0, n_events = 0, n_split = 0;
   int2* q = nullptr;
   size_t qcap = 0;
   int* qcls = nullptr;
@@ -1207,6 +1252,7 @@ static StepView ctx_view(const chp_ctx* ctx) {
   v.n_sap = ctx->n_sap;
   v.n_sleep = ctx->n_sleep;
   v.n_events = ctx->n_events;
+  v.n_split = ctx->n_split;
   v.q = ctx->d_q;
   v.qcap = ctx->d_q_cap;
   v.qcls = ctx->d_qcls;
@@ -1295,6 +1341,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
   if (V.nw) k_big_rowscan<R><<<V.nw, 1024, 0, st>>>(dw, d_list, V.bigw());
   k_pair_rows<R, ROWS_AB><<<qblocks, 64, 0, st>>>(dw, V.q, qbounds, dt, 0);
   k_joint_rows<R, ROWS_AB><<<n, 64, 0, st>>>(dw, d_list, dt, 0);
+  if (V.n_split) k_split_prep<R><<<n, 1024, 0, st>>>(dw, d_list);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[3], st));
   if (tune) {
     for (int c = 0; c < 3; c++) {
@@ -1322,6 +1369,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
     k_gs<R, true><<<groups(bg), bg, 0, st>>>(dw, d_list, nbig, n, bcap, keep);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
   }
+  if (V.n_split) k_split_gs<R><<<n, 256, 0, st>>>(dw, d_list, keep);
   if (V.nw) k_big_post<R><<<V.nspans, 64, 0, st>>>(dw, d_list, V.spans(), dt);
   k_post<R><<<n, 256, 0, st>>>(dw, d_list, dt);
   if (ev) HIPCHK(ctx, hipEventRecord(ev[7], st));
@@ -1344,8 +1392,8 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
   *out = nullptr;
   if (!(g.valid && g.list == list && g.nbig == nbig && g.max_nb == max_nb && g.dt == dt && g.bl == ctx->levels_block && g.keep == ctx->keep_rows &&
         g.bg == ctx->solve_block && g.dq == ctx->d_q && g.dqcls == ctx->d_qcls && g.dlist == ctx->d_list &&
-        g.dbig == ctx->d_big && g.big == std::array<int, 7>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep,
-                                       ctx->n_events} &&
+        g.dbig == ctx->d_big && g.big == std::array<int, 8>{ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep,
+                                       ctx->n_events, ctx->n_split} &&
         g.dworlds == ctx->d_worlds)) {
     if (!may_capture) return CHP_OK;
     drop_graph(ctx);
@@ -1364,7 +1412,8 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
     g.valid = true; g.list = list; g.nbig = nbig; g.max_nb = max_nb; g.dt = dt;
     g.bl = ctx->levels_block; g.bg = ctx->solve_block; g.keep = ctx->keep_rows;
     g.dq = ctx->d_q; g.dqcls = ctx->d_qcls; g.dlist = ctx->d_list; g.dworlds = ctx->d_worlds;
-    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep, ctx->n_events};
+    g.dbig = ctx->d_big; g.big = {ctx->big_nspans, ctx->big_ntiles, ctx->big_nw, ctx->big_rounds, ctx->n_sap, ctx->n_sleep, ctx->n_events,
+             ctx->n_split};
     g.graph = graph;
     g.exec = exec;
   }
@@ -1548,6 +1597,7 @@ template <typename R> static int step_noalloc_t(chp_ctx* ctx, uint32_t id, doubl
   V.n_sap = S.broadphase == CHP_BROADPHASE_SAP ? 1 : 0;
   V.n_sleep = S.may_sleep ? 1 : 0;
   V.n_events = S.ev.mask ? 1 : 0;
+  V.n_split = S.split ? 1 : 0;
   V.qcls = (int*)(base + o_qcls);
   V.q = (int2*)(base + o_q);
   V.qcap = (size_t)S.pair_cap;
@@ -1583,6 +1633,12 @@ template <typename R> static int set_capacity_t(chp_ctx* ctx, uint32_t id, uint3
 #define G(field) if (e == hipSuccess) e = get(H.field, D.field)
   G(pos); G(quat); G(vel); G(avel); G(force); G(torque); G(iiw); G(aabb_lo); G(aabb_hi); G(flags); G(order);
   G(sleep_state); G(sleep_tls); G(ray_state); G(prev_pos); G(prev_quat); G(interp_pos); G(interp_quat);
+  if (H.wd.solver == CHP_SOLVER_SPLIT) {   // the id pools at the slab's current sizes
+    H.sp_ctl.resize(4);
+    H.sp_cpool.resize(S.contact_cap);
+    H.sp_fpool.resize(2 * (size_t)S.contact_cap);
+    G(sp_ctl); G(sp_cpool); G(sp_fpool);
+  }
 #undef G
   if (e == hipSuccess) e = hipMemcpy(&H.time, D.wtime, sizeof(double), hipMemcpyDeviceToHost);
   int counters[C_COUNT];
@@ -2521,6 +2577,8 @@ int chp_solve(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t n, do
   std::lock_guard<std::mutex> g(ctx->mu);
   if (!valid_world(ctx, id) || (n && !eqs) || !iterations) return fail(ctx, CHP_E_INVALID, "bad argument");
   if (!(dt > 0)) return fail(ctx, CHP_E_INVALID, "dt must be > 0");
+  if (ctx->worlds[id].split)
+    return fail(ctx, CHP_E_INVALID, "the solver slot solves with GSSolver; a SplitSolver world's solver runs in its step");
   (void)hipSetDevice(ctx->device);
   if (!std::isfinite(tolerance)) return fail(ctx, CHP_E_INVALID, "tolerance must be finite");
   return ctx->precision == CHP_PRECISION_F64

@@ -116,6 +116,10 @@ template <typename R> struct HostWorld {
   std::vector<int> cm_table;
   // constraints
   std::vector<int> c_type, c_a, c_b, c_collide, c_row_off, c_eq_mask;
+  // Equation.id of each joint row: the joints' equations are numbered in creation order
+  // (constraint by constraint, each in its equations list order) before any step
+  std::vector<int> c_row_id;
+  int neq_ids = 0;   // the equations the joints created (the next id)
   std::vector<float> c_vec;
   std::vector<R> c_eq;
   int ncrows = 0;
@@ -125,6 +129,8 @@ template <typename R> struct HostWorld {
   int sap_axis = 0;
   double time = 0;   // World.time (kept on the host when the slab is rebuilt)
   std::vector<int> ray_state = std::vector<int>(4, 0);   // the world's ray state (kept likewise)
+  // SplitSolver's id counter / pool sizes and pools (kept likewise)
+  std::vector<int> sp_ctl, sp_cpool, sp_fpool;
 };
 
 // Shape.calculateWorldAABB for one body-shape instance, in FP64 (box.dart:149-192,
@@ -549,10 +555,12 @@ template <typename R> struct Builder {
         }
         default: fail("unknown constraint type");
       }
-      (void)nslots;
       bool trig = (W.flags[d.body_a] & BF_TRIGGER) || (W.flags[d.body_b] & BF_TRIGGER);
       int nrows = trig ? 0 : __builtin_popcount(mask);
       if (trig) mask = 0;
+      for (int k = 0; k < nslots; k++)
+        if (mask & (1 << k)) W.c_row_id.push_back(W.neq_ids + k);
+      W.neq_ids += nslots;
       bool collide = d.collide_connected < 0 ? true : d.collide_connected != 0;
       if (!collide) W.has_filtered = true;
       W.c_type.push_back(d.type); W.c_a.push_back((int)d.body_a); W.c_b.push_back((int)d.body_b);
@@ -593,12 +601,14 @@ template <typename R> struct Builder {
     if (S.world.broadphase != CHP_BROADPHASE_NAIVE && S.world.broadphase != CHP_BROADPHASE_SAP)
       fail("unknown broadphase");
     if (S.world.iterations < 0) fail("negative iteration count");
+    if (S.world.solver != CHP_SOLVER_GS && S.world.solver != CHP_SOLVER_SPLIT) fail("unknown solver");
     materials();
     heightfields();
     shapes();
     build_bodies();
     sap_axis();
     constraints();
+    if (S.world.solver == CHP_SOLVER_SPLIT) W.sp_ctl = {W.neq_ids, 0, 0, 0};   // ids go on from the joints'
   }
 };
 

@@ -1754,7 +1754,7 @@ template <typename R> __device__ bool post_body(const DevWorld<R>& W, int i, R d
   ((f4*)W.prev_quat)[i] = ((const f4*)W.quat)[i];
   f3 v = F3<R>(W.vel, i), w = F3<R>(W.avel, i);
   f3 lf = F3<R>(W.lin_factor, i), af = F3<R>(W.ang_factor, i);
-  if (E != 0) {
+  if (E != 0 && (W.solver_kind != CHP_SOLVER_SPLIT || (W.flags[i] & BF_SOLVED))) {
     v = add3<R>(v, mul3<R>(F3<R>(W.vl, i), lf));
     w = add3<R>(w, mul3<R>(F3<R>(W.wl, i), af));
   }

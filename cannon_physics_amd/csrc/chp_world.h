@@ -33,6 +33,7 @@ enum : int {
   BF_ISOTROPIC = 8,      // invInertia.x == .y == .z (updateInertiaWorld skips)
   BF_ALLOW_SLEEP = 16,   // Body.allowSleep
   BF_WAKE = 32,          // Body.wakeUpAfterNarrowphase
+  BF_SOLVED = 64,        // SplitSolver: in an island with equations (its velocity takes vlambda)
 };
 
 // control words of a world's event bookkeeping: key-set parity, the sizes of the four key sets
@@ -75,7 +76,8 @@ template <typename R> struct DevWorld {
   // parameters
   int broadphase, sap_axis, iterations, quat_skip;
   int quat_fast, has_filtered_constraints, stepnumber, allow_sleep;   // allow_sleep: World.allowSleep
-  int time_frozen, pad_tf;   // 1: the steps do not advance World.time (internalStep under step(dt, t))
+  int time_frozen;   // 1: the steps do not advance World.time (internalStep under step(dt, t))
+  int solver_kind;   // CHP_SOLVER_*
   float gravity[3];
   float pad1;
   R fric_grav_len, tol2;
@@ -138,6 +140,11 @@ template <typename R> struct DevWorld {
   CHP_G(R) sbr;
   CHP_G(int) counters;   // [0] status [1] pairs [2] contacts [3] rows [4] iterations [5] levels [6] staging
   CHP_G(int) ray_state;  // the world's ray: the heightfield cell index its first call kept
+  // SplitSolver: the joint rows' equation ids; the id counter and pool sizes (sp_ctl), the pools
+  // of contact and friction equation ids; per step: new ids, scans, row ids, island union-find,
+  // the sort of rows by island and id
+  CHP_G(int) c_row_id, sp_ctl, sp_cpool, sp_fpool, sp_newc, sp_newf, sp_scan, row_id, sp_parent, sp_vals;
+  CHP_G(unsigned long long) sp_keys;
   // contact events (null / 0 when off): the overlap keepers' sorted keys (body and shape sets,
   // two of each: the previous step's and the current one, by parity), two sort buffers of kcap
   // keys, scan scratch, the justTest overlaps of the step (bi, bj, si, sj) and the event buffer

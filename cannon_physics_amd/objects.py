@@ -331,9 +331,22 @@ class SAPBroadphase:
 
 
 class GSSolver:
+    kind = A.SOLVER_GS
+
     def __init__(self, tolerance=1e-7, iterations=10):
         self.tolerance = float(tolerance)
         self.iterations = int(iterations)
+
+
+class SplitSolver:
+    """SplitSolver(subsolver) (split_solver.dart:25-47): GSSolver per island; its own iterations
+    and tolerance (Solver's defaults, 10 and 1e-7) are handed to the subsolver at every solve."""
+    kind = A.SOLVER_SPLIT
+
+    def __init__(self, subsolver=None):
+        self.subsolver = subsolver if subsolver is not None else GSSolver()
+        self.iterations = 10
+        self.tolerance = 1e-7
 
 
 class Constraint:
@@ -495,6 +508,7 @@ def compile_scene(world):
         wd.sap_auto_axis = int(world.broadphase.auto_axis)
     wd.iterations = world.solver.iterations
     wd.tolerance = world.solver.tolerance
+    wd.solver = world.solver.kind
     wd.quat_normalize_skip = world.quatNormalizeSkip
     wd.quat_normalize_fast = int(world.quatNormalizeFast)
     cm = world.defaultContactMaterial

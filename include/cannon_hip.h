@@ -43,6 +43,9 @@ enum { CHP_SHAPE_SPHERE = 0, CHP_SHAPE_PLANE = 1, CHP_SHAPE_BOX = 2, CHP_SHAPE_C
 /* BodyTypes, lib/objects/rigid_body.dart:15 */
 enum { CHP_BODY_DYNAMIC = 0, CHP_BODY_STATIC = 1, CHP_BODY_KINEMATIC = 2 };
 enum { CHP_BROADPHASE_NAIVE = 0, CHP_BROADPHASE_SAP = 1 };
+/* GSSolver; SplitSolver(GSSolver) (split_solver.dart): GSSolver per island of bodies connected
+ * by equations, the island's equations by descending Equation.id */
+enum { CHP_SOLVER_GS = 0, CHP_SOLVER_SPLIT = 1 };
 enum { CHP_CONSTRAINT_POINT_TO_POINT = 0, CHP_CONSTRAINT_HINGE = 1, CHP_CONSTRAINT_LOCK = 2,
        CHP_CONSTRAINT_DISTANCE = 3, CHP_CONSTRAINT_CONE_TWIST = 4 };
 /* BodySleepStates, lib/objects/rigid_body.dart */
@@ -70,7 +73,8 @@ typedef struct chp_world_desc {          /* World({...}) world_class.dart:135-16
   double friction, restitution;
   double contact_stiffness, contact_relaxation, friction_stiffness, friction_relaxation;
   int32_t allow_sleep;                   /* World.allowSleep (default 0) */
-  int32_t reserved;
+  int32_t solver;                        /* CHP_SOLVER_*: World.solver's class; iterations and
+                                            tolerance above are that solver's */
 } chp_world_desc;
 
 typedef struct chp_material_desc {       /* Material, material.dart:19-25 */

@@ -390,6 +390,7 @@ World* build_world(const chp_scene* sc) {
   w->quatNormalizeSkip = wd.quat_normalize_skip;
   w->quatNormalizeFast = wd.quat_normalize_fast != 0;
   w->allowSleep = wd.allow_sleep != 0;
+  w->solverKind = wd.solver;
   w->defaultCM.friction = wd.friction;
   w->defaultCM.restitution = wd.restitution;
   w->defaultCM.contactEquationStiffness = wd.contact_stiffness;
@@ -514,6 +515,9 @@ World* build_world(const chp_scene* sc) {
     else w->sapAxis = 2;
   }
   for (uint32_t c = 0; c < sc->num_constraints; c++) build_constraint(*w, sc->constraints[c]);
+  /* the constraints' equations are created with the constraints, before any step */
+  for (auto& c : w->constraints)
+    for (Equation* e : c->equations) e->id = w->nextEquationId++;
   w->shapeBody.assign(sc->num_shapes, -1);   /* Body.addShape sets shape.body: the last body wins */
   for (Body* b : w->bodies)
     for (Shape* s : b->shapes) w->shapeBody[s->id] = b->index;

@@ -112,7 +112,11 @@ struct NP {
   /* Narrowphase.createContactEquation, narrow_phase.dart:492-528 */
   Equation& createContact(Body* bi, Body* bj, SRef si, SRef sj, const SRef* rsi, const SRef* rsj) {
     if (justTest) throw JustTestHit();   /* `if (justTest) return true;` before each contact */
+    int id;   /* contactPointPool.removeLast(), else a new ContactEquation */
+    if (!w.contactPool.empty()) { id = w.contactPool.back(); w.contactPool.pop_back(); }
+    else id = w.nextEquationId++;
     w.contacts.emplace_back();
+    w.contacts.back().id = id;
     Equation& c = w.contacts.back();
     c.kind = CHP_ROW_CONTACT;
     c.bi = bi; c.bj = bj;
@@ -150,6 +154,10 @@ struct NP {
       R reducedMass = c.bi->invMass + c.bj->invMass;
       if (reducedMass > (R)0) reducedMass = (R)1 / reducedMass;
       Equation f1, f2;
+      for (Equation* f : {&f1, &f2}) {   /* frictionEquationPool.removeLast(), else new */
+        if (!w.frictionPool.empty()) { f->id = w.frictionPool.back(); w.frictionPool.pop_back(); }
+        else f->id = w.nextEquationId++;
+      }
       f1.kind = f2.kind = CHP_ROW_FRICTION;
       f1.bi = f2.bi = c.bi;
       f1.bj = f2.bj = c.bj;
@@ -1012,13 +1020,14 @@ static void updateSolveMass(Body& b) {
 
 /* GSSolver.solve, gs_solver.dart:27-133 */
 static int gs_solve(World& w, std::vector<Equation*>& eqs, R h, std::vector<R>& Bs, std::vector<R>& invCs,
-                    std::vector<R>& lambda) {
+                    std::vector<R>& lambda, const std::vector<Body*>* bodyList = nullptr) {
+  const std::vector<Body*>& bodies = bodyList ? *bodyList : w.bodies;   /* world.bodies */
   int iter = 0;
   int maxIter = w.iterations;
   R tolSquared = w.tolerance * w.tolerance;
   size_t nEq = eqs.size();
   if (nEq != 0)
-    for (Body* b : w.bodies) updateSolveMass(*b);
+    for (Body* b : bodies) updateSolveMass(*b);
   Bs.assign(nEq, 0); invCs.assign(nEq, 0); lambda.assign(nEq, 0);
   for (size_t i = 0; i != nEq; i++) {
     Equation& c = *eqs[i];
@@ -1027,7 +1036,7 @@ static int gs_solve(World& w, std::vector<Equation*>& eqs, R h, std::vector<R>& 
     invCs[i] = (R)1 / (computeGiMGt(c) + c.eps);
   }
   if (nEq != 0) {
-    for (Body* b : w.bodies) { setValues(b->vlambda, 0, 0, 0); setValues(b->wlambda, 0, 0, 0); }
+    for (Body* b : bodies) { setValues(b->vlambda, 0, 0, 0); setValues(b->wlambda, 0, 0, 0); }
     for (iter = 0; iter != maxIter; iter++) {
       R tot = 0;
       for (size_t j = 0; j != nEq; j++) {
@@ -1043,7 +1052,7 @@ static int gs_solve(World& w, std::vector<Equation*>& eqs, R h, std::vector<R>& 
       }
       if (tot * tot < tolSquared) break;
     }
-    for (Body* b : w.bodies) {
+    for (Body* b : bodies) {
       multiply2(b->vlambda, b->linearFactor, b->vlambda);
       add2(b->velocity, b->vlambda, b->velocity);
       multiply2(b->wlambda, b->angularFactor, b->wlambda);
@@ -1053,6 +1062,69 @@ static int gs_solve(World& w, std::vector<Equation*>& eqs, R h, std::vector<R>& 
     for (size_t l = 0; l < nEq; l++) eqs[l]->multiplier = lambda[l] * invDt;
   }
   return iter;
+}
+
+/* SplitSolver.solve (split_solver.dart:50-120): islands of non-static bodies found from the
+ * first unvisited one in body order by the reference's traversal (a stack), each island's
+ * equations (first seen first, then sorted by descending Equation.id) solved by GSSolver on a
+ * world of the island's bodies.  Returns the number of islands; Bs / invCs / lambda are
+ * returned per equation of eqs. */
+static int split_solve(World& w, std::vector<Equation*>& eqs, R h, std::vector<R>& Bs, std::vector<R>& invCs,
+                       std::vector<R>& lambda) {
+  const int nb = (int)w.bodies.size();
+  struct Node { std::vector<int> children; std::vector<Equation*> eqs; bool visited = false; };
+  std::vector<Node> nodes(nb);
+  for (Equation* eq : eqs) {
+    int i = eq->bi->index, j = eq->bj->index;
+    nodes[i].children.push_back(j); nodes[i].eqs.push_back(eq);
+    nodes[j].children.push_back(i); nodes[j].eqs.push_back(eq);
+  }
+  auto unvisited = [&](const std::vector<int>& list) {  /* getUnvisitedNode, :122-131 */
+    for (int k : list)
+      if (!nodes[k].visited && w.bodies[k]->type != CHP_BODY_STATIC) return k;
+    return -1;
+  };
+  std::vector<int> all(nb);
+  for (int k = 0; k < nb; k++) all[k] = k;
+  std::map<const Equation*, size_t> where;
+  for (size_t k = 0; k < eqs.size(); k++) where[eqs[k]] = k;
+  Bs.assign(eqs.size(), 0); invCs.assign(eqs.size(), 0); lambda.assign(eqs.size(), 0);
+  int n = 0;
+  while (true) {
+    int root = unvisited(all);
+    if (root < 0) break;
+    std::vector<Equation*> ieqs;
+    std::vector<Body*> ibodies;
+    auto visit = [&](int k) {  /* visitFunc, :155-164 */
+      ibodies.push_back(w.bodies[k]);
+      for (Equation* e : nodes[k].eqs)
+        if (std::find(ieqs.begin(), ieqs.end(), e) == ieqs.end()) ieqs.push_back(e);
+    };
+    std::vector<int> queue;  /* bfs, :136-153 */
+    queue.push_back(root);
+    nodes[root].visited = true;
+    visit(root);
+    while (!queue.empty()) {
+      int node = queue.back();
+      queue.pop_back();
+      while (true) {
+        int child = unvisited(nodes[node].children);
+        if (child < 0) break;
+        nodes[child].visited = true;
+        visit(child);
+        queue.push_back(child);
+      }
+    }
+    std::sort(ieqs.begin(), ieqs.end(), [](const Equation* a, const Equation* b) { return a->id > b->id; });
+    std::vector<R> b_, c_, l_;
+    gs_solve(w, ieqs, h, b_, c_, l_, &ibodies);
+    for (size_t k = 0; k < ieqs.size(); k++) {
+      size_t at = where[ieqs[k]];
+      Bs[at] = b_[k]; invCs[at] = c_[k]; lambda[at] = l_[k];
+    }
+    n++;
+  }
+  return n;
 }
 
 /* Constraint.update of each class */
@@ -1202,12 +1274,19 @@ void internal_step(World& w, R dt) {
   }
   w.cap.pi.clear(); w.cap.pj.clear();
   for (size_t k = 0; k < p1.size(); k++) { w.cap.pi.push_back(p1[k]->index); w.cap.pj.push_back(p2[k]->index); }
+  /* the last step's contact and friction equations go back to their pools, :506-519 */
+  for (int id : w.lastContactIds) w.contactPool.push_back(id);
+  for (int id : w.lastFrictionIds) w.frictionPool.push_back(id);
   w.contacts.clear();
   w.frictions.clear();
   w.contacts.reserve(4 * p1.size() + 16);
   NP np(w);
   np.getContacts(p1, p2);
   w.cap.contacts = (int)w.contacts.size();
+  w.lastContactIds.clear();
+  w.lastFrictionIds.clear();
+  for (const Equation& c : w.contacts) w.lastContactIds.push_back(c.id);
+  for (const Equation& f : w.frictions) w.lastFrictionIds.push_back(f.id);
   /* Solver.addEquation filter, solver.dart:30-34 */
   std::vector<Equation*> eqs;
   auto add = [&](Equation* e) { if (e->enabled && !e->bi->isTrigger && !e->bj->isTrigger) eqs.push_back(e); };
@@ -1280,7 +1359,8 @@ void internal_step(World& w, R dt) {
     for (Equation* e : c->equations) add(e);
   }
   std::vector<R> Bs, invCs, lambda;
-  int iters = gs_solve(w, eqs, dt, Bs, invCs, lambda);
+  int iters = w.solverKind == CHP_SOLVER_SPLIT ? split_solve(w, eqs, dt, Bs, invCs, lambda)
+                                              : gs_solve(w, eqs, dt, Bs, invCs, lambda);
   w.cap.iterations = iters;
   w.cap.rows.clear();
   for (size_t i = 0; i < eqs.size(); i++) w.cap.rows.push_back(capture_row(*eqs[i], Bs[i], invCs[i], lambda[i]));
@@ -1306,6 +1386,13 @@ void internal_step(World& w, R dt) {
 void narrowphase_equations(World& w, const std::vector<int>& pi, const std::vector<int>& pj, R dt,
                            std::vector<Equation>& out, int& nf, int& nc) {
   w.dt = dt;
+  /* the slot leaves the World's equation pools and id counter as they were */
+  struct Keep {
+    World& w;
+    std::vector<int> cp, fp;
+    int next;
+    ~Keep() { w.contactPool = cp; w.frictionPool = fp; w.nextEquationId = next; }
+  } pools{w, w.contactPool, w.frictionPool, w.nextEquationId};
   std::vector<Body*> p1, p2;
   for (size_t k = 0; k < pi.size(); k++) { p1.push_back(w.bodies[pi[k]]); p2.push_back(w.bodies[pj[k]]); }
   w.contacts.clear();

@@ -106,6 +106,7 @@ struct Equation {
   R targetVelocity = 0;
   /* events only: shapes recorded on the contact */
   int shapeMatA = -1, shapeMatB = -1;   /* (c.si.material, c.sj.material) for friction */
+  int id = -1;                          /* Equation.id (equation_class.dart:34-37) */
   int si = -1, sj = -1;                 /* ContactEquation.si / .sj (scene shape indices) */
 };
 
@@ -185,6 +186,12 @@ struct World {
   bool allowSleep = false;   /* World.allowSleep */
   double time = 0;           /* World.time */
   double accumulator = 0;    /* World.accumulator */
+  int solverKind = CHP_SOLVER_GS;
+  /* Equation.idCounter and the World's equation pools (world_class.dart:501-519, contacts and
+   * friction equations reused last in, first out): the ids the reused objects carry */
+  int nextEquationId = 0;
+  std::vector<int> contactPool, frictionPool;
+  std::vector<int> lastContactIds, lastFrictionIds;   /* the last step's equations, in order */
   ContactMat defaultCM;
   std::vector<MaterialDesc> materials;
   std::map<std::pair<int, int>, ContactMat> cmTable;
