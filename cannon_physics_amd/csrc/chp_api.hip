@@ -351,9 +351,63 @@ __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ 
   }
 }
 
-// GS iterations.  GLOBAL = false: the worlds whose body velocities and level starts fit in
-// LDS; GLOBAL = true: the others, from global memory (a separate kernel, so that the common
-// LDS form keeps its own, smaller register count)
+// GS sweeps of the worlds of at most GS_SUB_MAX_NB bodies run G lanes per world, 64 / G worlds
+// per wave (a level of such a world is rarely wider than a few dozen rows, and the sweep is a
+// chain of dependent steps: packing worlds into a wave divides the instructions issued for the
+// same rows by 64 / G).  Their vlambda / wlambda and level starts live in LDS.
+#ifndef CHP_GS_SUB_MAX_NB
+#define CHP_GS_SUB_MAX_NB 512
+#endif
+constexpr int GS_SUB_MAX_NB = CHP_GS_SUB_MAX_NB;
+__host__ __device__ constexpr int gs_group(int nb) { return nb <= 64 ? 16 : nb <= 128 ? 32 : nb <= GS_SUB_MAX_NB ? 64 : 0; }
+__host__ __device__ constexpr int gs_sub_nb(int G) { return G == 16 ? 64 : G == 32 ? 128 : GS_SUB_MAX_NB; }
+__host__ __device__ constexpr int gs_sub_ls(int G) { return G == 16 ? 512 : G == 32 ? 1024 : 2048; }
+constexpr size_t gs_sub_bytes(int G) {
+  return (size_t)(64 / G) * (8 * gs_sub_nb(G) * sizeof(float) + gs_sub_ls(G) * sizeof(int));
+}
+// whether k_gs_sub runs the world (else k_gs does)
+template <typename R> __device__ __forceinline__ bool gs_sub_fits(const DevWorld<R>& W, int L) {
+  const int G = gs_group(W.nb);
+  return G != 0 && L + 1 <= gs_sub_ls(G);
+}
+
+// the list positions of a list sorted by body count (largest first): [0, blk) over
+// GS_SUB_MAX_NB bodies, [blk, s32) G = 64, [s32, s16) G = 32, [s16, n) G = 16
+struct GsSegs {
+  int blk = 0, s32 = 0, s16 = 0;
+};
+
+// the group's world: list position lo + (block b of the segment) * (64 / G) + group
+template <typename R, int G>
+__device__ __forceinline__ void gs_sub_run(const DevWorld<R>* worlds, const int* list, int lo, int hi, int b,
+                                           float* lds, int keep) {
+  const int sub = ((int)threadIdx.x & 63) / G;
+  const int idx = lo + b * (64 / G) + sub;
+  if (idx >= hi) return;
+  const DevWorld<R>& W = worlds[list[idx]];
+  if (W.counters[C_STATUS] != 0 || W.solver_kind == CHP_SOLVER_SPLIT) return;
+  if (gs_group(W.nb) != G || !gs_sub_fits(W, W.counters[C_LEVELS])) return;
+  float* base = lds + sub * (8 * gs_sub_nb(G) + gs_sub_ls(G));
+  st_gs<R, GSub<G>, true>(W, GSub<G>(), nullptr, nullptr, base, (int*)(base + 8 * gs_sub_nb(G)), keep != 0);
+}
+// one wave per workgroup; the workgroups are dealt to the three segments (G = 64 first: their
+// worlds have the longest chains), so that all of them run in one launch
+template <typename R>
+__global__ __launch_bounds__(64) void k_gs_sub(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                               GsSegs g, int n, int keep) {
+  extern __shared__ float4 lds_f4[];
+  float* lds = (float*)lds_f4;
+  const int b = (int)blockIdx.x;
+  const int n64 = g.s32 - g.blk, n32 = (g.s16 - g.s32 + 1) / 2;
+  if (b < n64) gs_sub_run<R, 64>(worlds, list, g.blk, g.s32, b, lds, keep);
+  else if (b < n64 + n32) gs_sub_run<R, 32>(worlds, list, g.s32, g.s16, b - n64, lds, keep);
+  else gs_sub_run<R, 16>(worlds, list, g.s16, n, b - n64 - n32, lds, keep);
+}
+
+// GS iterations of the rest.  GLOBAL = false: the worlds over GS_SUB_MAX_NB bodies whose body
+// velocities and level starts fit in LDS, one per workgroup; GLOBAL = true: every world that
+// neither this nor k_gs_sub runs, from global memory (a separate kernel, so that the common LDS
+// form keeps its own, smaller register count)
 template <typename R, bool GLOBAL>
 __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
                                             int nbig, int n, int big_cap, int keep) {
@@ -375,14 +429,27 @@ __global__ __launch_bounds__(256) void k_gs(const DevWorld<R>* __restrict__ worl
       if (!fits) st_gs<R, GBlock, false>(W, GBlock(), shd, sh, W.vw, nullptr, keep != 0);
     }
   } else {
-    const bool fits = W.nb <= SMALL_NB && L + 1 <= LS_SMALL;
-    float* base = lds + wave * (8 * SMALL_NB + LS_SMALL);
-    if constexpr (!GLOBAL) {
-      if (fits) st_gs<R, GWave, true>(W, GWave(), shd, sh, base, (int*)(base + 8 * SMALL_NB), keep != 0);
-    } else {
-      if (!fits) st_gs<R, GWave, false>(W, GWave(), shd, sh, W.vw, nullptr, keep != 0);
+    if constexpr (GLOBAL) {
+      if (!gs_sub_fits(W, L)) st_gs<R, GWave, false>(W, GWave(), shd, sh, W.vw, nullptr, keep != 0);
     }
   }
+}
+
+// the GS stage of a list (the level schedule and the GS records are in place)
+template <typename R>
+static void launch_gs(const DevWorld<R>* dw, const int* d_list, int n, const GsSegs& g, int bcap, int bd, int keep,
+                      hipStream_t st) {
+  if (g.blk) k_gs<R, false><<<g.blk, bd, lds_gs(bd, bcap), st>>>(dw, d_list, g.blk, g.blk, bcap, keep);
+  const int nsub = (g.s32 - g.blk) + (g.s16 - g.s32 + 1) / 2 + (n - g.s16 + 3) / 4;
+  if (nsub) {
+    size_t bytes = 0;
+    if (g.s32 > g.blk) bytes = std::max(bytes, gs_sub_bytes(64));
+    if (g.s16 > g.s32) bytes = std::max(bytes, gs_sub_bytes(32));
+    if (n > g.s16) bytes = std::max(bytes, gs_sub_bytes(16));
+    k_gs_sub<R><<<nsub, 64, bytes, st>>>(dw, d_list, g, n, keep);
+  }
+  const int per = bd / 64;
+  k_gs<R, true><<<g.blk + (n - g.blk + per - 1) / per, bd, 0, st>>>(dw, d_list, g.blk, n, bcap, keep);
 }
 
 // World.step(dt, timeSinceLastCalled): the interpolated poses and World.time += the elapsed time
@@ -666,6 +733,7 @@ struct chp_ctx {
   int n_sleep = 0; // worlds in the current list that may hold sleeping bodies
   int n_events = 0; // worlds in the current list with contact events on
   int n_split = 0;  // worlds in the current list with a SplitSolver
+  GsSegs gsg;       // the GS kernels' segments of the current list
   bool keep_rows = false;   // steps also write row records (the capture reads them)
   int* d_status = nullptr;
   const int** d_cptrs = nullptr;
@@ -1182,6 +1250,16 @@ static int upload_list(chp_ctx* ctx, const std::vector<int>& list) {
   }
   ctx->h_list = list;
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, ctx->h_list.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  {  // lists are sorted by body count, largest first (or hold one world)
+    auto lead = [&](int nb_over) {
+      size_t k = 0;
+      while (k < n && (int)ctx->worlds[list[k]].nb > nb_over) k++;
+      return (int)k;
+    };
+    ctx->gsg.blk = lead(GS_SUB_MAX_NB);
+    ctx->gsg.s32 = lead(128);
+    ctx->gsg.s16 = lead(64);
+  }
   std::vector<int> spans, tiles, bigw;
   int max_tiles = 1;
   ctx->n_sap = 0;
@@ -1236,6 +1314,7 @@ struct StepView {
   const int* list = nullptr;
   const int* big = nullptr;
   int nspans = 0, ntiles = 0, nw = 0, rounds = 0, n_sap = 0, n_sleep = 0, n_events = 0, n_split = 0;
+  GsSegs gsg;
   int2* q = nullptr;
   size_t qcap = 0;
   int* qcls = nullptr;
@@ -1253,6 +1332,7 @@ static StepView ctx_view(const chp_ctx* ctx) {
   v.n_sleep = ctx->n_sleep;
   v.n_events = ctx->n_events;
   v.n_split = ctx->n_split;
+  v.gsg = ctx->gsg;
   v.q = ctx->d_q;
   v.qcap = ctx->d_q_cap;
   v.qcls = ctx->d_qcls;
@@ -1350,8 +1430,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
       k_levels<R><<<groups(b), b, lds_levels(b, bcap), st>>>(dw, d_list, nbig, n, bcap, 0);
       build_rows();   // at the positions this schedule gave (the order inside a level may differ)
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 1], st));
-      k_gs<R, false><<<groups(b), b, lds_gs(b, bcap), st>>>(dw, d_list, nbig, n, bcap, keep);
-      k_gs<R, true><<<groups(b), b, 0, st>>>(dw, d_list, nbig, n, bcap, keep);
+      launch_gs<R>(dw, d_list, n, V.gsg, bcap, b, keep, st);
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 2], st));
     }
     if (ev) {
@@ -1365,8 +1444,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
     if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
     build_rows();
     if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
-    k_gs<R, false><<<groups(bg), bg, lds_gs(bg, bcap), st>>>(dw, d_list, nbig, n, bcap, keep);
-    k_gs<R, true><<<groups(bg), bg, 0, st>>>(dw, d_list, nbig, n, bcap, keep);
+    launch_gs<R>(dw, d_list, n, V.gsg, bcap, bg, keep, st);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
   }
   if (V.n_split) k_split_gs<R><<<n, 256, 0, st>>>(dw, d_list, keep);
@@ -1598,6 +1676,9 @@ template <typename R> static int step_noalloc_t(chp_ctx* ctx, uint32_t id, doubl
   V.n_sleep = S.may_sleep ? 1 : 0;
   V.n_events = S.ev.mask ? 1 : 0;
   V.n_split = S.split ? 1 : 0;
+  V.gsg.blk = S.nb > GS_SUB_MAX_NB ? 1 : 0;
+  V.gsg.s32 = S.nb > 128 ? 1 : 0;
+  V.gsg.s16 = S.nb > 64 ? 1 : 0;
   V.qcls = (int*)(base + o_qcls);
   V.q = (int2*)(base + o_q);
   V.qcap = (size_t)S.pair_cap;
@@ -2039,8 +2120,7 @@ static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t 
   k_solve_rows<R><<<1, 256, 0, st>>>(dw, (int)id, ctx->d_eq, (int)n, dt);
   const int bcap = big_lds_cap(nbig, S.nb, ctx->max_lds);
   k_levels<R><<<1, bl, lds_levels(bl, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap, 1);
-  k_gs<R, false><<<1, bg, lds_gs(bg, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap, 1);
-  k_gs<R, true><<<1, bg, 0, st>>>(dw, ctx->d_list, nbig, 1, bcap, 1);
+  launch_gs<R>(dw, ctx->d_list, 1, ctx->gsg, bcap, bg, 1, st);
   k_solve_finish<R><<<1, 256, 0, st>>>(dw, (int)id);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(st));

@@ -1195,6 +1195,7 @@ struct GBlock {
     return r;
   }
   __device__ int exscan(int* data, int cnt, int* sh) const { return array_exscan(data, cnt, sh); }
+  __device__ int uni(int v) const { return __builtin_amdgcn_readfirstlane(v); }
 };
 // one wave on one world: LDS and memory accesses of a wave are issued in program order, so a
 // fence that waits for them replaces the barrier
@@ -1227,6 +1228,27 @@ struct GWave {
     }
     sync();
     return carry;
+  }
+  __device__ int uni(int v) const { return __builtin_amdgcn_readfirstlane(v); }
+};
+// G lanes of one wave on one world (G = 16, 32 or 64), 64 / G worlds per wave.  A group's lanes
+// always take the same path (every branch of the GS sweep depends on the world only), so the
+// wave's program order orders the group's memory accesses as it does a whole wave's; values
+// uniform over the group are not uniform over the wave unless G = 64.
+template <int G> struct GSub {
+  int tid, n;
+  __device__ GSub() : tid((int)threadIdx.x & (G - 1)), n(G) {}
+  __device__ void sync() const { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+  __device__ void lds_sync() const { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local"); }
+  __device__ double sum(double v, double*) const {
+    // fixed butterfly inside the group: every lane of it ends with the same value
+    for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+  }
+  __device__ int bcast0(int v, int*) const { return __shfl(v, ((int)threadIdx.x & 63) & ~(G - 1), 64); }
+  __device__ int uni(int v) const {
+    if constexpr (G == 64) return __builtin_amdgcn_readfirstlane(v);
+    else return v;
   }
 };
 
@@ -1465,8 +1487,10 @@ __device__ __forceinline__ R gs_row(const GsRow<R>& cr, R clam, bool v, int ks, 
   f3 vb = mk3(vb4.x, vb4.y, vb4.z), wb = mk3(wb4.x, wb4.y, wb4.z);
   R gwl = mulvec<R>(va, wa, cr.sa, cr.ra) + mulvec<R>(vb, wb, sb, cr.rb);
   R dl = cr.invC * (cr.B - gwl - cr.eps * clam);
-  if (clam + dl < cr.minF) dl = cr.minF - clam;
-  else if (clam + dl > cr.maxF) dl = cr.maxF - clam;
+  // the clamp of gs_solver.dart:91-96 as selects (a branch per lane costs exec-mask switches on
+  // the sweep's dependent chain); NaN compares false, as in the branches
+  const R t = clam + dl;
+  dl = t < cr.minF ? cr.minF - clam : (t > cr.maxF ? cr.maxF - clam : dl);
   R lnew = clam + dl;
   R adlv = dl > (R)0 ? dl : -dl;
   lam[ks] = lnew;
@@ -1496,8 +1520,9 @@ __device__ __forceinline__ R gs_row(const GsRow<R>& cr, R clam, bool v, int ks, 
 // The sweep is a sequence of steps, one level of one iteration each.  A thread's first row of
 // each level (position ls[l] + tid) is loaded D steps ahead into a ring of registers, so the
 // row stream's memory latency overlaps D levels; further rows of a level wider than the group
-// are loaded when they are reached.  A row recurs after L steps: when L < D a ring slot may hold
-// a row whose lambda changes before the slot is used, and the new lambda is forwarded to it.
+// are loaded when they are reached.  A sweep is padded to a whole number of ring turns, so a row
+// recurs exactly one padded sweep later, in the same slot: when the sweep is a single turn the
+// slot's next row is the row itself, and its new lambda is forwarded in registers.
 // lam / adl have one dummy slot at row_cap for idle lanes.
 #ifndef CHP_GS_DEPTH_F32
 #define CHP_GS_DEPTH_F32 2   // ring depth D of the FP32 sweep (2 measured a few per cent faster than 4)
@@ -1506,7 +1531,7 @@ template <typename R, typename G, bool LDS>
 __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* shd, int* sh, float* vw, int* ls_lds,
                                       bool keep) {
   constexpr int D = sizeof(R) == 4 ? CHP_GS_DEPTH_F32 : 2;
-  const int E = uni(W.counters[C_ROWS]), L = uni(W.counters[C_LEVELS]), iters = uni(W.iterations);
+  const int E = g.uni(W.counters[C_ROWS]), L = g.uni(W.counters[C_LEVELS]), iters = g.uni(W.iterations);
   const int nb = W.nb, tid = g.tid, bd = g.n, dummy = W.row_cap;
   const R tol2 = W.tol2;
   const int* ls = W.lvl_start;
@@ -1538,7 +1563,7 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
       const int c = ls[l + 1] - ls[l];
       lp += c > bd ? (c + bd - 1) / bd : 1;
     }
-    const int Lp = uni((int)g.sum((double)lp, shd));
+    const int Lp = g.uni((int)g.sum((double)lp, shd));
     // A sweep is padded with idle steps (no row on any thread) to a multiple of the ring depth,
     // so that a sweep always ends after the ring's last slot: the loop then has one exit test
     // per turn, and a loop with one exit keeps the wait counts exact (exits after each slot make
@@ -1573,6 +1598,11 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
         int k;
         next(k, qv[s]);
         qk[s] = qv[s] ? k : 0;
+        // the two stores a step issues before its slot's loads (to the dummy slot here), so that
+        // the loop entered from here holds the memory operations in flight that it holds after
+        // a turn: the waits at its head then cover only the slot the step uses
+        lam[dummy] = (R)0;
+        adl[dummy] = (R)0;
         gs_load<R>(gsr, qk[s], q[s]);
         ql[s] = lam[qk[s]];
         qf[s] = (R)0;
@@ -1587,16 +1617,13 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
       for (;;) {
 #pragma unroll
         for (int s = 0; s < D; s++) {
+          // nothing of another slot's step may move into this one: an instruction that reads a
+          // slot's row before its own step makes the wave wait for that row's loads, and the
+          // ring then hides no latency
+          __builtin_amdgcn_sched_barrier(0);
           const int k = qk[s];
           const bool v = qv[s];
           const R lnew = gs_row<R>(q[s], qo[s] ? qf[s] : ql[s], v, v ? k : dummy, vw, lam, adl, part);
-          // forwarding touches only qf / qo: a select on ql would wait for loads still in flight
-#pragma unroll
-          for (int s2 = 0; s2 < D; s2++)
-            if (s2 != s && v && qv[s2] && qk[s2] == k) {
-              qf[s2] = lnew;
-              qo[s2] = true;
-            }
           {  // the slot's next row (D steps on)
             int kn;
             bool vn;
@@ -1617,7 +1644,7 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
           step = 0;
           const double Dsum = g.sum(part, shd);
           part = 0.0;
-          dec = uni(gs_decide<R>(Dsum, E, tol2));
+          dec = g.uni(gs_decide<R>(Dsum, E, tol2));
           if (dec != 0) break;
           if (++it == iters) {
             dec = 1;
@@ -1635,7 +1662,7 @@ __device__ __forceinline__ void st_gs(const DevWorld<R>& W, const G& g, double* 
         for (int r = 0; r < E; r++) tot += adl[rpos[r]];
         vv = tot * tot < tol2 ? 1 : 0;
       }
-      if (uni(g.bcast0(vv, sh)) || ++it == iters) break;
+      if (g.uni(g.bcast0(vv, sh)) || ++it == iters) break;
       // next sweep: the ring starts again at its first step
     }
     g.sync();
