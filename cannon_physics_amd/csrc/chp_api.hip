@@ -303,7 +303,16 @@ static int big_lds_cap(int nbig, int max_nb, size_t max_lds) {
   return nbig ? std::max(BIG_NB_LDS_MIN, std::min(max_nb, most)) : BIG_NB_LDS_MIN;
 }
 
-// dynamic LDS of k_levels / k_gs for a workgroup of bd threads (big_cap: as big_lds_cap)
+// bodies of a large world whose level walk keeps each body's last level in LDS (4 bytes per
+// body): up to the largest large world of the launch, at most what a workgroup's LDS holds beside
+// the level histogram and the static arrays (4 KiB kept free)
+static int levels_lds_cap(int nbig, int max_nb, size_t max_lds) {
+  const int most = (int)((max_lds - 4096 - (size_t)LS_BIG * sizeof(int)) / sizeof(int));
+  return nbig ? std::max(BIG_NB_LDS_MIN, std::min(max_nb, most)) : BIG_NB_LDS_MIN;
+}
+
+// dynamic LDS of k_levels / k_gs for a workgroup of bd threads (big_cap: as levels_lds_cap /
+// big_lds_cap)
 static size_t lds_levels(int bd, int big_cap) {
   size_t big = (size_t)(big_cap + LS_BIG) * sizeof(int), small = (size_t)(SMALL_NB + LS_SMALL) * sizeof(int);
   return std::max(big, (bd / 64) * small);
@@ -1370,7 +1379,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
   const DevWorld<R>* dw = (const DevWorld<R>*)V.worlds;
   const int* d_list = V.list;
   auto groups = [&](int bd) { return nbig + (n - nbig + bd / 64 - 1) / (bd / 64); };
-  const int bcap = big_lds_cap(nbig, max_nb, ctx->max_lds);
+  const int bcap = big_lds_cap(nbig, max_nb, ctx->max_lds), lcap = levels_lds_cap(nbig, max_nb, ctx->max_lds);
   hipStream_t st = ctx->stream;
   if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
   {
@@ -1427,7 +1436,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
     for (int c = 0; c < 3; c++) {
       const int b = kTuneBlocks[c];
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c], st));
-      k_levels<R><<<groups(b), b, lds_levels(b, bcap), st>>>(dw, d_list, nbig, n, bcap, 0);
+      k_levels<R><<<groups(b), b, lds_levels(b, lcap), st>>>(dw, d_list, nbig, n, lcap, 0);
       build_rows();   // at the positions this schedule gave (the order inside a level may differ)
       HIPCHK(ctx, hipEventRecord(ctx->tev[3 * c + 1], st));
       launch_gs<R>(dw, d_list, n, V.gsg, bcap, b, keep, st);
@@ -1440,7 +1449,7 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
     }
   } else {
     const int bl = ctx->levels_block, bg = ctx->solve_block;
-    k_levels<R><<<groups(bl), bl, lds_levels(bl, bcap), st>>>(dw, d_list, nbig, n, bcap, 0);
+    k_levels<R><<<groups(bl), bl, lds_levels(bl, lcap), st>>>(dw, d_list, nbig, n, lcap, 0);
     if (ev) HIPCHK(ctx, hipEventRecord(ev[4], st));
     build_rows();
     if (ev) HIPCHK(ctx, hipEventRecord(ev[5], st));
@@ -2118,8 +2127,8 @@ static int solve_t(chp_ctx* ctx, uint32_t id, const chp_equation* eqs, uint32_t 
   const int bl = ctx->levels_block, bg = ctx->solve_block, nbig = S.nb > SMALL_NB ? 1 : 0;
   hipStream_t st = ctx->stream;
   k_solve_rows<R><<<1, 256, 0, st>>>(dw, (int)id, ctx->d_eq, (int)n, dt);
-  const int bcap = big_lds_cap(nbig, S.nb, ctx->max_lds);
-  k_levels<R><<<1, bl, lds_levels(bl, bcap), st>>>(dw, ctx->d_list, nbig, 1, bcap, 1);
+  const int bcap = big_lds_cap(nbig, S.nb, ctx->max_lds), lcap = levels_lds_cap(nbig, S.nb, ctx->max_lds);
+  k_levels<R><<<1, bl, lds_levels(bl, lcap), st>>>(dw, ctx->d_list, nbig, 1, lcap, 1);
   launch_gs<R>(dw, ctx->d_list, 1, ctx->gsg, bcap, bg, 1, st);
   k_solve_finish<R><<<1, 256, 0, st>>>(dw, (int)id);
   HIPCHK(ctx, hipGetLastError());

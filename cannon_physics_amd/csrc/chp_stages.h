@@ -1369,15 +1369,24 @@ __device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int*
   int lmax = 0;
   if (tid < 64) {
     const int lane = tid;
+    // the runs' bounds and body pairs are loaded ahead (bounds two chunks, pairs one chunk), so
+    // the walk does not wait for them
+    auto load_rs = [&](int c) { const int q = c + lane; return q < NR ? make_int2(rstart[q], rstart[q + 1]) : make_int2(0, 0); };
+    auto load_ab = [&](int2 rs, int c) { return c + lane < NR ? ab[rs.x] : make_int2(0, 0); };
+    int2 rs_next = load_rs(0);
+    int2 x_next = load_ab(rs_next, 0);
+    int2 rs_after = load_rs(64);
     for (int c0 = 0; c0 < NR; c0 += 64) {
       const int q = c0 + lane;
       const bool act = q < NR;
+      const int2 rs = rs_next, x = x_next;
+      x_next = load_ab(rs_after, c0 + 64);
+      rs_next = rs_after;
+      rs_after = load_rs(c0 + 128);
       int a = 0, b = 0, len = 0;
       bool da = false, db = false;
       if (act) {
-        int s0 = rstart[q];
-        int2 x = ab[s0];
-        len = rstart[q + 1] - s0;
+        len = rs.y - rs.x;
         da = x.x & GS_DYN;
         db = x.y & GS_DYN;
         a = x.x & GS_IDX;

@@ -115,3 +115,40 @@ def test_large_world_pair_overflow_is_reported_and_recoverable():
     finally:
         c.destroy_world(wid)
         c.destroy_world(ref)
+
+
+def test_full_size_sap_world_mid_run_bit_exact():
+    """The 32,768-body SAP world at its real size, mid-run (the pile forming): device and oracle
+    step 150 steps from construction (the SAP axis list keeps its order from step to step, so
+    both run the same history) with bit-identical state; one captured step then gives the same
+    pairs, rows, iteration count and state, and two more steps the same state
+    (sap_broadphase.dart:138-190, gs_solver.dart:27-133)."""
+    rec = scenes.c2_large_world().recipe()
+    c = ctx("f32")
+    wid = c.create_world(rec)
+    try:
+        print("[large] c2 32768 f32: 150 steps on both sides", flush=True)
+        ow = O.OracleWorld(rec, "f32")
+        c.step([wid], DT, 150)
+        ow.step(DT, 150)
+        g, o = state_to_numpy(*c.download(wid)), ow.state()
+        bad = [k for k in g if not np.array_equal(bits(g[k]), bits(o[k]))]
+        assert not bad, bad
+        print("[large] c2 32768 f32: captured step", flush=True)
+        pi, pj, rows, it = c.step_capture(wid, DT, pair_cap=1 << 20, row_cap=1 << 20)
+        ow.step(DT, 1)
+        opi, opj = ow.pairs(cap=1 << 20)
+        assert len(pi) > 100000 and len(pi) == len(opi)
+        diff = np.nonzero((pi != opi) | (pj != opj))[0]
+        assert len(diff) == 0, (len(diff), diff[:4], pi[diff[:4]], pj[diff[:4]], opi[diff[:4]], opj[diff[:4]])
+        orow = ow.rows(cap=1 << 20)
+        assert len(rows["kind"]) == len(orow["kind"]) > 100000
+        for k in rows:
+            assert np.array_equal(bits(rows[k]), bits(orow[k])), k
+        assert it == ow.iterations()
+        assert same(state_to_numpy(*c.download(wid)), ow.state())
+        c.step([wid], DT, 2)
+        ow.step(DT, 2)
+        assert same(state_to_numpy(*c.download(wid)), ow.state())
+    finally:
+        c.destroy_world(wid)
