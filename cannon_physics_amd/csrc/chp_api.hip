@@ -294,6 +294,7 @@ __global__ __launch_bounds__(64) void k_joint_rows(const DevWorld<R>* __restrict
 // and one small world (at most SMALL_NB bodies) per wave, blockDim / 64 of them per workgroup.
 constexpr int SMALL_NB = 128;        // bodies of a world run by one wave
 constexpr int LS_BIG = 2048, LS_SMALL = 1024;   // level starts kept in LDS
+constexpr int LS_LEVELS = 4096;                  // the level histogram of a large world's level stage
 // bodies of a large world whose GS / level state fits in LDS: at least 512, up to the largest
 // large world of the launch (the launch's dynamic LDS is sized for it), at most what a
 // workgroup's LDS holds beside the level starts and the static arrays (4 KiB kept free)
@@ -307,14 +308,14 @@ static int big_lds_cap(int nbig, int max_nb, size_t max_lds) {
 // body): up to the largest large world of the launch, at most what a workgroup's LDS holds beside
 // the level histogram and the static arrays (4 KiB kept free)
 static int levels_lds_cap(int nbig, int max_nb, size_t max_lds) {
-  const int most = (int)((max_lds - 4096 - (size_t)LS_BIG * sizeof(int)) / sizeof(int));
+  const int most = (int)((max_lds - 4096 - (size_t)LS_LEVELS * sizeof(int)) / sizeof(int));
   return nbig ? std::max(BIG_NB_LDS_MIN, std::min(max_nb, most)) : BIG_NB_LDS_MIN;
 }
 
 // dynamic LDS of k_levels / k_gs for a workgroup of bd threads (big_cap: as levels_lds_cap /
 // big_lds_cap)
 static size_t lds_levels(int bd, int big_cap) {
-  size_t big = (size_t)(big_cap + LS_BIG) * sizeof(int), small = (size_t)(SMALL_NB + LS_SMALL) * sizeof(int);
+  size_t big = (size_t)(big_cap + LS_LEVELS) * sizeof(int), small = (size_t)(SMALL_NB + LS_SMALL) * sizeof(int);
   return std::max(big, (bd / 64) * small);
 }
 static size_t lds_gs(int bd, int big_cap) {
@@ -351,7 +352,7 @@ __global__ __launch_bounds__(256) void k_levels(const DevWorld<R>* __restrict__ 
   const DevWorld<R>& W = worlds[id];
   if (W.solver_kind == CHP_SOLVER_SPLIT) return;   // k_split_prep orders its rows
   if (wave < 0) {
-    if (W.nb <= big_cap) st_levels<R, GBlock, true>(W, GBlock(), sh, lds_i, lds_i + big_cap, LS_BIG, gather != 0);
+    if (W.nb <= big_cap) st_levels<R, GBlock, true>(W, GBlock(), sh, lds_i, lds_i + big_cap, LS_LEVELS, gather != 0);
     else st_levels<R, GBlock, false>(W, GBlock(), sh, W.tmp, nullptr, 0, gather != 0);
   } else {
     int* base = lds_i + wave * (SMALL_NB + LS_SMALL);

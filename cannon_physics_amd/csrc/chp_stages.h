@@ -1404,11 +1404,14 @@ __device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int*
         if (db) lev = max(lev, atomicAdd(&last[b], 0) + 1);
       }
       const int n = uni(min(64, NR - c0));
+      // branch-free: a branch per lane costs exec-mask switches on this loop's dependent chain
+      int endv = lev + span - 1;
       for (int j = 0; j < n - 1; j++) {
-        int kaj = __builtin_amdgcn_readlane(ka, j), kbj = __builtin_amdgcn_readlane(kb, j);
-        int endj = __builtin_amdgcn_readlane(lev + span - 1, j);
-        bool hit = lane > j && (ka == kaj || ka == kbj || kb == kaj || kb == kbj);
-        lev = hit ? max(lev, endj + 1) : lev;
+        const int kaj = __builtin_amdgcn_readlane(ka, j), kbj = __builtin_amdgcn_readlane(kb, j);
+        const int endj = __builtin_amdgcn_readlane(endv, j);
+        const int hit = (int)(lane > j) & ((int)(ka == kaj) | (int)(ka == kbj) | (int)(kb == kaj) | (int)(kb == kbj));
+        lev = max(lev, hit ? endj + 1 : 0);
+        endv = lev + span - 1;
       }
       // the chain through a body has increasing levels: the chunk's last run on it wins
       if (da) atomicMax(&last[a], lev + span - 1);

@@ -278,9 +278,11 @@ int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, doubl
 int chp_sync(chp_ctx* ctx);
 /* device time (HIP events on the context stream) of the last chp_step / chp_step_plain call */
 int chp_last_step_ms(chp_ctx* ctx, double* ms);
-/* launch shape of the GS and level stages: workgroup sizes (64, 128 or 256; 0 keeps).  A large
- * world takes a workgroup, small worlds (<= 128 bodies) take one wave each.  Results do not
- * depend on the choice (the GS schedule is exact whatever the grouping). */
+/* launch shape of the GS and level stages: workgroup sizes (64, 128 or 256; 0 keeps).  The level
+ * stage gives a world of over 128 bodies a workgroup and smaller worlds one wave each; the GS
+ * stage gives a world of over 512 bodies a workgroup (solve_block threads) and smaller worlds
+ * 16, 32 or 64 lanes each (fixed).  Results do not depend on the choice (the GS schedule is exact
+ * whatever the grouping). */
 int chp_ctx_set_tuning(chp_ctx* ctx, int solve_block, int levels_block);
 /* Autotuner: the next waited chp_step runs the level and GS stages of its first step once per
  * workgroup size (64, 128, 256), times them with HIP events and keeps the fastest of each
