@@ -714,6 +714,29 @@ struct WorldSlot {
 
 }  // namespace
 
+// The per-list state of a batched call: the stream its launches go to, its world list and
+// large-world tables, its narrowphase queue and GS segments.  The context's own fields hold the
+// state of its stream; `second` holds a second stream's, which run_steps swaps in to launch half
+// of a large batch through the same code, so that the two halves' latency-bound stage tails
+// overlap on the device.
+struct ListState {
+  hipStream_t stream = nullptr;
+  int* d_list = nullptr;
+  size_t d_list_cap = 0;
+  std::vector<int> h_list;
+  bool list_valid = false;
+  int* d_big = nullptr;
+  size_t d_big_cap = 0;
+  std::vector<int> h_big;
+  int big_nspans = 0, big_ntiles = 0, big_nw = 0, big_rounds = 0;
+  int n_sap = 0, n_sleep = 0, n_events = 0, n_split = 0;
+  GsSegs gsg;
+  int2* d_q = nullptr;
+  size_t d_q_cap = 0;
+  int* d_qcls = nullptr;
+  size_t d_qcls_cap = 0;
+};
+
 struct chp_ctx {
   int device = 0;
   int precision = CHP_PRECISION_F32;
@@ -744,6 +767,8 @@ struct chp_ctx {
   int n_events = 0; // worlds in the current list with contact events on
   int n_split = 0;  // worlds in the current list with a SplitSolver
   GsSegs gsg;       // the GS kernels' segments of the current list
+  ListState second;          // the second stream's list state (see ListState)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool keep_rows = false;   // steps also write row records (the capture reads them)
   int* d_status = nullptr;
   const int** d_cptrs = nullptr;
@@ -798,6 +823,37 @@ struct chp_ctx {
   uint64_t next_ticket = 1;
   int lds_cap_bodies = 2048;
 };
+
+// the context's list state <-> the second stream's
+static void swap_lists(chp_ctx* ctx) {
+  ListState& o = ctx->second;
+  std::swap(ctx->stream, o.stream);
+  std::swap(ctx->d_list, o.d_list);
+  std::swap(ctx->d_list_cap, o.d_list_cap);
+  std::swap(ctx->h_list, o.h_list);
+  std::swap(ctx->list_valid, o.list_valid);
+  std::swap(ctx->d_big, o.d_big);
+  std::swap(ctx->d_big_cap, o.d_big_cap);
+  std::swap(ctx->h_big, o.h_big);
+  std::swap(ctx->big_nspans, o.big_nspans);
+  std::swap(ctx->big_ntiles, o.big_ntiles);
+  std::swap(ctx->big_nw, o.big_nw);
+  std::swap(ctx->big_rounds, o.big_rounds);
+  std::swap(ctx->n_sap, o.n_sap);
+  std::swap(ctx->n_sleep, o.n_sleep);
+  std::swap(ctx->n_events, o.n_events);
+  std::swap(ctx->n_split, o.n_split);
+  std::swap(ctx->gsg, o.gsg);
+  std::swap(ctx->d_q, o.d_q);
+  std::swap(ctx->d_q_cap, o.d_q_cap);
+  std::swap(ctx->d_qcls, o.d_qcls);
+  std::swap(ctx->d_qcls_cap, o.d_qcls_cap);
+}
+// a world was created, destroyed or re-slabbed: both lists' device tables are out of date
+static void invalidate_lists(chp_ctx* ctx) {
+  ctx->list_valid = false;
+  ctx->second.list_valid = false;
+}
 
 static int fail(chp_ctx* c, int code, const std::string& m) {
   if (c) c->err = m;
@@ -966,7 +1022,7 @@ static int build_slot(chp_ctx* ctx, WorldSlot& S, const HostWorld<R>& H, int pai
     if (r != CHP_OK) return r;
   }
   ctx->worlds_dirty = true;
-  ctx->list_valid = false;
+  invalidate_lists(ctx);
   return CHP_OK;
 }
 
@@ -1028,7 +1084,7 @@ static void release_world(chp_ctx* ctx, uint32_t id) {
   events_free(ctx, S);
   S = WorldSlot();
   ctx->worlds_dirty = true;
-  ctx->list_valid = false;
+  invalidate_lists(ctx);
 }
 
 static bool valid_world(chp_ctx* ctx, uint32_t id) { return ctx && id < ctx->worlds.size() && ctx->worlds[id].live; }
@@ -1074,7 +1130,7 @@ template <typename R> static void events_attach(chp_ctx* ctx, WorldSlot& S) {
     D.ev_mask = (int)S.ev.mask; D.ev_cap = S.ev.cap; D.ev_kcap = S.ev.kcap;
   }
   ctx->worlds_dirty = true;
-  ctx->list_valid = false;
+  invalidate_lists(ctx);
 }
 // buffers for the world's current capacities and the given event capacity; a live allocation
 // that is too small moves into the new one (key sets, control words, pending events)
@@ -1511,15 +1567,76 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
 
 // launches nsteps steps of the listed worlds (largest first); with wait, synchronises and, when
 // status is given, returns every listed world's status word in list order
+// batches of at least this many worlds run as two halves on two streams
+constexpr uint32_t SPLIT_MIN_WORLDS = 64;
+
+static int status_words(chp_ctx* ctx, const uint32_t* ids, uint32_t n, std::vector<int>* status) {
+  std::vector<int> st;
+  int r = read_status(ctx, ids, n, st);
+  if (r != CHP_OK) return r;
+  status->resize(n);
+  for (uint32_t k = 0; k < n; k++) (*status)[k] = st[k * C_COUNT + C_STATUS];
+  return CHP_OK;
+}
+
+// run_steps for a large batch: the worlds of the size-sorted list are dealt alternately to two
+// halves, launched step by step on the two streams (the second forked from the first after the
+// world tables are uploaded, and joined to it at the end), so that each half's latency-bound stage
+// tails (the level walk, the GS sweeps of its deepest worlds) overlap with the other half's work.
+// Worlds are independent, so every world's result is the one it gets alone.
+template <typename R>
+static int run_split(chp_ctx* ctx, const std::vector<int>& list, double dt, int nsteps, bool wait,
+                     const uint32_t* ids, uint32_t n, std::vector<int>* status) {
+  std::vector<int> part[2];
+  for (size_t k = 0; k < list.size(); k++) part[k & 1].push_back(list[k]);
+  int nbig[2], max_nb[2];
+  for (int h = 0; h < 2; h++) {
+    nbig[h] = 0;
+    while (nbig[h] < (int)part[h].size() && ctx->worlds[part[h][nbig[h]]].nb > SMALL_NB) nbig[h]++;
+    max_nb[h] = ctx->worlds[part[h][0]].nb;
+    std::vector<uint32_t> pid(part[h].begin(), part[h].end());
+    if (h) swap_lists(ctx);
+    int r = ensure_queue(ctx, pid.data(), (uint32_t)pid.size());
+    if (r == CHP_OK) r = upload_list(ctx, part[h]);
+    if (h) swap_lists(ctx);
+    if (r != CHP_OK) return r;
+  }
+  ctx->prev_list = list;
+  ctx->prev_dt = dt;
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->second.stream, ctx->ev_fork, 0));
+  for (int s = 0; s < nsteps; s++) {
+    int r = launch_step<R>(ctx, ctx_view(ctx), (int)part[0].size(), nbig[0], max_nb[0], dt, nullptr);
+    if (r != CHP_OK) return r;
+    swap_lists(ctx);
+    r = launch_step<R>(ctx, ctx_view(ctx), (int)part[1].size(), nbig[1], max_nb[1], dt, nullptr);
+    swap_lists(ctx);
+    if (r != CHP_OK) return r;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev_join, ctx->second.stream));
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  ctx->timed = true;
+  ctx->stage_ms.assign(K_NSTAGE, 0.0);
+  if (!wait) return CHP_OK;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return status ? status_words(ctx, ids, n, status) : CHP_OK;
+}
+
 template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, uint32_t n, double dt, int nsteps,
                                            bool wait, std::vector<int>* status) {
   int r = prepare<R>(ctx, ids, n, dt);
   if (r != CHP_OK) return r;
-  r = ensure_queue(ctx, ids, n);
-  if (r != CHP_OK) return r;
   // largest worlds first so that the long workgroups start early
   std::vector<int> list(ids, ids + n);
   std::stable_sort(list.begin(), list.end(), [&](int a, int b) { return ctx->worlds[a].nb > ctx->worlds[b].nb; });
+  // profiled, autotuned and graph-replayed calls keep to one stream
+  if (n >= SPLIT_MIN_WORLDS && nsteps > 0 && !ctx->profile && !(ctx->tune_pending && wait) &&
+      !(ctx->use_graphs && nsteps > 1))
+    return run_split<R>(ctx, list, dt, nsteps, wait, ids, n, status);
+  r = ensure_queue(ctx, ids, n);
+  if (r != CHP_OK) return r;
   int nbig = 0;
   while (nbig < (int)n && ctx->worlds[list[nbig]].nb > SMALL_NB) nbig++;
   r = upload_list(ctx, list);
@@ -1581,13 +1698,7 @@ template <typename R> static int run_steps(chp_ctx* ctx, const uint32_t* ids, ui
         HIPCHK(ctx, hipEventElapsedTime(&f, ctx->sev[s * (K_NSTAGE + 1) + k], ctx->sev[s * (K_NSTAGE + 1) + k + 1]));
         ctx->stage_ms[k] += f;
       }
-  if (status) {
-    std::vector<int> st;
-    r = read_status(ctx, ids, n, st);
-    if (r != CHP_OK) return r;
-    status->resize(n);
-    for (uint32_t k = 0; k < n; k++) (*status)[k] = st[k * C_COUNT + C_STATUS];
-  }
+  if (status) return status_words(ctx, ids, n, status);
   return CHP_OK;
 }
 
@@ -1805,7 +1916,7 @@ template <typename R> static int state_io(chp_ctx* ctx, uint32_t id, chp_body_st
     const bool may = D.allow_sleep != 0 || asleep;
     if (may != S.may_sleep) {   // the step's wake-up pass is launched per list
       S.may_sleep = may;
-      ctx->list_valid = false;
+      invalidate_lists(ctx);
     }
     int zero = 0;  // a fresh state clears a quarantine
     HIPCHK(ctx, hipMemcpy(D.counters + C_STATUS, &zero, 4, hipMemcpyHostToDevice));
@@ -2338,6 +2449,13 @@ int chp_ctx_create(int device, uint64_t hbm_budget, int precision, chp_ctx** out
   c->max_lds = (size_t)prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CHP_E_HIP; }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return CHP_E_HIP; }
+  // the second stream (default priority) and the events that fork it from and join it to the first
+  if (hipStreamCreateWithFlags(&c->second.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    delete c;
+    return CHP_E_HIP;
+  }
   c->scratch = scratch_address(device);   // the device is current; checked on use
 
   *out = c;
@@ -2360,6 +2478,14 @@ int chp_ctx_destroy(chp_ctx* ctx) {
   if (ctx->d_qbuf) (void)hipFree(ctx->d_qbuf);
   if (ctx->d_q) (void)hipFree(ctx->d_q);
   if (ctx->d_qcls) (void)hipFree(ctx->d_qcls);
+  (void)hipStreamSynchronize(ctx->second.stream);
+  if (ctx->second.d_list) (void)hipFree(ctx->second.d_list);
+  if (ctx->second.d_big) (void)hipFree(ctx->second.d_big);
+  if (ctx->second.d_q) (void)hipFree(ctx->second.d_q);
+  if (ctx->second.d_qcls) (void)hipFree(ctx->second.d_qcls);
+  (void)hipStreamDestroy(ctx->second.stream);
+  (void)hipEventDestroy(ctx->ev_fork);
+  (void)hipEventDestroy(ctx->ev_join);
   for (hipEvent_t e : ctx->sev) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
@@ -2393,7 +2519,7 @@ int chp_world_destroy(chp_ctx* ctx, uint32_t id) {
   events_free(ctx, S);
   S = WorldSlot();
   ctx->worlds_dirty = true;
-  ctx->list_valid = false;
+  invalidate_lists(ctx);
   return CHP_OK;
 }
 

@@ -121,6 +121,34 @@ def test_fp32_within_tolerance_of_reference_model():
         c.destroy_world(wid)
 
 
+def test_two_stream_batch_equals_one_stream():
+    """A batch of 64+ worlds runs as two halves on two streams (an unprofiled chp_step, and
+    chp_step_async + chp_sync); a profiled call runs the same batch on one stream: every world
+    ends bit-identical either way, with large (grid-wide) worlds and sleep/event worlds in it."""
+    c = ctx("f32")
+    recs = [scenes.c3_world(i).recipe() for i in range(200, 276)]
+    recs += [scenes.c2_large_world(seed=7, n=1200, extent=(20.0, 8.0, 20.0)).recipe(), scenes.sleep_world().recipe()]
+    two = [c.create_world(r) for r in recs]
+    one = [c.create_world(r) for r in recs]
+    asy = [c.create_world(r) for r in recs]
+    try:
+        c.step(two, DT, 30)
+        c.set_profiling(True)
+        c.step(one, DT, 30)
+        c.set_profiling(False)
+        for _ in range(3):
+            c.step_async(asy, DT, 10)
+        c.sync()
+        for a, b, d in zip(two, one, asy):
+            g1, g2, g3 = (state_to_numpy(*c.download(w)) for w in (a, b, d))
+            assert all(same_state(g1, g2).values())
+            assert all(same_state(g1, g3).values())
+            assert c.stats(a)["steps"] == c.stats(b)["steps"] == 30
+    finally:
+        for w in two + one + asy:
+            c.destroy_world(w)
+
+
 def test_batch_invariance_and_plain_equals_batched():
     """A world gives bit-identical results alone, in a batch of mixed worlds, and on the plain
     one-world-per-launch path."""
