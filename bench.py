@@ -182,9 +182,14 @@ def serve_bench(ntemplates, nrequests, budget_gib, tick_steps, seed=5, max_submi
             bsteps += nb * k
     ctx.close()
     lat = np.array(lat)
+    # the steady state: from one mean residence time after the first arrival to the last arrival
+    # (the ramp-up and the drain of the stream are left out)
+    last_arrival = int(arrivals[min(submitted, len(arrivals)) - 1]) if submitted else 0
+    steady = occ[int(mean_ticks): last_arrival + 1] or occ
     return {"requests": submitted, "completed": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "mean_wait_ms": float(np.mean(waits)),
             "body_steps_per_s": bsteps / wall, "wall_s": wall, "mean_occupancy": float(np.mean(occ)),
+            "steady_occupancy": float(np.mean(steady)), "steady_ticks": [int(mean_ticks), last_arrival],
             "hbm_budget_gib": budget_gib, "steps_per_tick": tick_steps,
             "config": "Poisson arrivals at %.2f requests per tick, %d request templates (log-uniform [16,2048] "
                       "bodies: spheres, boxes, 8-16 vertex hulls over a 64x64 heightfield), K ~ U{60..600} "
@@ -256,6 +261,7 @@ def precision_report(ctx, recs, wids, dyn, dt, preroll, steps_total, stride):
     c64.step(w64, dt, 1)
     dx = dv = dx_flip = 0.0
     flips = 0
+    per_world = []
     for a, b in zip(w32, w64):
         g, o = state_to_numpy(*ctx.download(a)), state_to_numpy(*c64.download(b))
         ex = float(np.max(np.abs(g["position"] - o["position"])))
@@ -265,6 +271,7 @@ def precision_report(ctx, recs, wids, dyn, dt, preroll, steps_total, stride):
             dx_flip = max(dx_flip, ex)
         else:
             dx, dv = max(dx, ex), max(dv, ev)
+            per_world.append(ex)
     for b in w64:
         c64.destroy_world(b)
     # drift over the whole run from the initial state, both precisions on the device
@@ -282,6 +289,9 @@ def precision_report(ctx, recs, wids, dyn, dt, preroll, steps_total, stride):
     c64.close()
     return {"worlds": len(sample), "after_steps": preroll,
             "one_step_max_abs_dx_m": dx, "one_step_max_abs_dv_m_per_s": dv,
+            "one_step_dx_median_m": float(np.median(per_world)) if per_world else 0.0,
+            "one_step_dx_p90_m": float(np.percentile(per_world, 90)) if per_world else 0.0,
+            "worlds_within_tolerance": int(sum(1 for e in per_world if e <= 1e-4)),
             "worlds_with_contact_count_flip": flips, "flip_worlds_max_abs_dx_m": dx_flip,
             "tolerance_m": 1e-4, "within_tolerance": dx <= 1e-4,
             "drift_steps": steps_total, "drift_max_abs_dx_m": drift,

@@ -249,10 +249,10 @@ template <typename R, int CLS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_narrow_wave(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
                                                      const int* __restrict__ bounds, double dt) {
   __shared__ f3 ws[4][NPCtx<R, true>::WS_F3];            // one LDS workspace per wave
-  __shared__ ContactRec<R> loc[4][CHP_KMAX];              // and one contact buffer
+  __shared__ ContactRec<R> loc[4][WAVE_KMAX];             // and one contact buffer
   constexpr unsigned kinds = CLS == NP_WHC ? (NK_CC | NK_HC) : NK_ALL;
   st_narrow_wave<R, kinds>(worlds, q + bounds[CLS], bounds[CLS + 1] - bounds[CLS], (R)dt, ws[threadIdx.x >> 6],
-                           loc[threadIdx.x >> 6]);
+                           loc[threadIdx.x >> 6], WAVE_KMAX);
 }
 
 

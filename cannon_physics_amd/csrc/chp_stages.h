@@ -768,11 +768,14 @@ __device__ void st_narrow_class(const DevWorld<R>* worlds, const int2* q, const 
   }
 }
 
+// contact records a wave buffers per pair: a heightfield pair touching several pillars has more
+// contacts than a thread's pair, and a pair over the buffer runs twice
+constexpr int WAVE_KMAX = 64;
 // the queued separating-axis pairs, one wave per pair (grid-stride over the queue)
-// (ws: the wave's LDS workspace; loc: its LDS contact buffer of CHP_KMAX records)
+// (ws: the wave's LDS workspace; loc: its LDS contact buffer of kmax records)
 template <typename R, unsigned KINDS>
 __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, R dt, f3* ws,
-                               ContactRec<R>* loc) {
+                               ContactRec<R>* loc, int kmax) {
   const int lane = threadIdx.x & 63;
   const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
   for (int i = gw; i < cnt; i += nw) {
@@ -780,7 +783,7 @@ __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int
     const DevWorld<R>& W = worlds[q.x];
     const int p = q.y;
     const int bi = W.pair_i[p], bj = W.pair_j[p];
-    Sink<R, true> s{loc, CHP_KMAX, 0, 0};
+    Sink<R, true> s{loc, kmax, 0, 0};
     NPCtx<R, true, KINDS> ctx(W, dt, &s, ws);
     ctx.pair(bi, bj);
     int n = s.n, base = 0;
@@ -792,7 +795,7 @@ __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int
     if (n > 0 && base + n > W.contact_cap) {
       if (lane == 0) ws_fail(W.counters, WS_CONTACTS);
       n = 0;
-    } else if (n > CHP_KMAX) {  // more contacts than the local buffer: run the pair again into the slab
+    } else if (n > kmax) {  // more contacts than the local buffer: run the pair again into the slab
       Sink<R, true> s2{W.contacts + base, n, 0, 0};
       NPCtx<R, true, KINDS> ctx2(W, dt, &s2, ws);
       ctx2.pair(bi, bj);
