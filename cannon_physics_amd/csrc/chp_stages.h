@@ -1353,19 +1353,46 @@ __device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int*
   int* rstart = W.rpos;     // first row of each run (+ E at the end), until the sort below
   int* rlev = W.sched;      // start level of each run, until the sort below
   for (int b = tid; b < nb; b += bd) last[b] = -1;
+  // The passes over the rows take LU rows per thread per turn, loading all of them before any
+  // store: each turn then waits for one round of loads rather than one per row.
+  constexpr int LU = 4;
   // runs: a row starts one when its body pair differs from the previous row's
-  for (int r = tid; r < E; r += bd) {
-    int2 x = ab[r];
-    runid[r] = (r == 0 || x.x != ab[r - 1].x || x.y != ab[r - 1].y) ? 1 : 0;
+  for (int r0 = tid; r0 < E; r0 += LU * bd) {
+    int2 x[LU], xp[LU];
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      const int r = r0 + u * bd;
+      x[u] = r < E ? ab[r] : make_int2(0, 0);
+      xp[u] = r > 0 && r < E ? ab[r - 1] : make_int2(-1, -1);
+    }
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      const int r = r0 + u * bd;
+      if (r < E) runid[r] = (r == 0 || x[u].x != xp[u].x || x[u].y != xp[u].y) ? 1 : 0;
+    }
   }
   g.sync();
   const int NR = uni(g.exscan(runid, E, sh));
-  for (int r = tid; r < E; r += bd) {
-    int2 x = ab[r];
-    bool first = r == 0 || x.x != ab[r - 1].x || x.y != ab[r - 1].y;
-    int id = runid[r] + (first ? 1 : 0) - 1;
-    runid[r] = id;
-    if (first) rstart[id] = r;
+  for (int r0 = tid; r0 < E; r0 += LU * bd) {
+    int2 x[LU], xp[LU];
+    int rid[LU];
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      const int r = r0 + u * bd;
+      x[u] = r < E ? ab[r] : make_int2(0, 0);
+      xp[u] = r > 0 && r < E ? ab[r - 1] : make_int2(-1, -1);
+      rid[u] = r < E ? runid[r] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      const int r = r0 + u * bd;
+      if (r < E) {
+        const bool first = r == 0 || x[u].x != xp[u].x || x[u].y != xp[u].y;
+        const int id = rid[u] + (first ? 1 : 0) - 1;
+        runid[r] = id;
+        if (first) rstart[id] = r;
+      }
+    }
   }
   if (tid == 0) rstart[NR] = E;
   g.sync();
@@ -1426,24 +1453,50 @@ __device__ __forceinline__ void st_levels(const DevWorld<R>& W, const G& g, int*
     }
   }
   const int L = uni(g.imax(lmax, sh));
-  for (int r = tid; r < E; r += bd) {
-    int id = runid[r], l0 = rlev[id];
-    runid[r] = l0 < 0 ? 0 : l0 + (r - rstart[id]);
+  for (int r0 = tid; r0 < E; r0 += LU * bd) {
+    int id[LU], l0[LU], rs[LU];
+#pragma unroll
+    for (int u = 0; u < LU; u++) id[u] = r0 + u * bd < E ? runid[r0 + u * bd] : 0;
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      l0[u] = rlev[id[u]];
+      rs[u] = rstart[id[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      const int r = r0 + u * bd;
+      if (r < E) runid[r] = l0[u] < 0 ? 0 : l0[u] + (r - rs[u]);
+    }
   }
   g.sync();
   // counting sort by level; the order inside a level is free (its rows are independent)
   auto sort = [&](int* cnt) {
     for (int l = tid; l <= L; l += bd) cnt[l] = 0;
     g.sync();
-    for (int r = tid; r < E; r += bd) atomicAdd(&cnt[W.level[r]], 1);
+    for (int r0 = tid; r0 < E; r0 += LU * bd) {
+      int lv[LU];
+#pragma unroll
+      for (int u = 0; u < LU; u++) lv[u] = r0 + u * bd < E ? W.level[r0 + u * bd] : -1;
+#pragma unroll
+      for (int u = 0; u < LU; u++)
+        if (lv[u] >= 0) atomicAdd(&cnt[lv[u]], 1);
+    }
     g.sync();
     g.exscan(cnt, L + 1, sh);
     for (int l = tid; l <= L; l += bd) W.lvl_start[l] = cnt[l];
     g.sync();
-    for (int r = tid; r < E; r += bd) {
-      int k = atomicAdd(&cnt[W.level[r]], 1);
-      W.sched[k] = r;
-      W.rpos[r] = k;
+    for (int r0 = tid; r0 < E; r0 += LU * bd) {
+      int lv[LU];
+#pragma unroll
+      for (int u = 0; u < LU; u++) lv[u] = r0 + u * bd < E ? W.level[r0 + u * bd] : -1;
+#pragma unroll
+      for (int u = 0; u < LU; u++)
+        if (lv[u] >= 0) {
+          const int r = r0 + u * bd;
+          const int k = atomicAdd(&cnt[lv[u]], 1);
+          W.sched[k] = r;
+          W.rpos[r] = k;
+        }
     }
     g.sync();
   };
