@@ -749,6 +749,45 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     if (!hfWindow(hf, local, radius, x0, x1, y0, y1, mn, mx)) return;
     if (ld<R>(local.z) - radius > mx || ld<R>(local.z) + radius < mn) return;
     SRef pref{1, -1, hfId};   // the pillar hull; its contacts name the heightfield
+    if constexpr (WAVE) {
+      // The window's pillars (in the loop order i, j, lower/upper) are dealt to the lanes, 64 a
+      // round.  A lane drops its pillar when the bounding spheres miss, or when one of the face
+      // axes that findSeparatingAxis tests (the hull's face 0, then every pillar face;
+      // convex_polyhedron.dart:252-330) separates the two: the search then answers
+      // "separated" whatever the edge axes say, and convexConvex adds nothing.  The same
+      // projections of the same vertices decide it here, so the pillars left are exactly those
+      // that can add contacts; the wave takes them in loop order through convexConvex.
+      const int lane = (int)(threadIdx.x & 63);
+      const int nyw = y1 - y0, ncand = (x1 - x0) * nyw * 2;
+      const f3 zero = mk3(0.f, 0.f, 0.f);
+      const f3 loA = point_to_local<R>(xi, qi, zero);
+      for (int c0 = 0; c0 < ncand; c0 += 64) {
+        const int c = c0 + lane;
+        bool live = false;
+        if (c < ncand) {
+          const int up = c & 1, ij = c >> 1, i = x0 + ij / nyw, j = y0 + ij % nyw;
+          const HullView<R> H = hull_view(W, pillarHull(hf, i, j, up));
+          const f3 wpo = point_to_world<R>(xj, qj, pillarOff(hf, i, j, up));
+          if (dist3<R>(xi, wpo) < H.bsr + A.bsr) {
+            live = true;
+            const f3 loB = point_to_local<R>(wpo, qj, zero);
+            R d;
+            if (A.axes && !testSepAxis(A, qrot<R>(qi, A.fn[0]), H, loA, qi, loB, qj, d)) live = false;
+            if (H.axes)
+              for (int f = 0; live && f < H.nf; f++)
+                if (!testSepAxis(A, qrot<R>(qj, H.fn[f]), H, loA, qi, loB, qj, d)) live = false;
+          }
+        }
+        for (uint64_t m = __ballot(live); m; m &= m - 1) {
+          const int cc = c0 + __builtin_ctzll(m);
+          const int up = cc & 1, ij = cc >> 1, i = x0 + ij / nyw, j = y0 + ij % nyw;
+          const HullView<R> H = hull_view(W, pillarHull(hf, i, j, up));
+          const f3 wpo = point_to_world<R>(xj, qj, pillarOff(hf, i, j, up));
+          convexConvex(A, siRef, H, pref, xi, wpo, qi, qj, bi, bj, siRef, pref, true);
+        }
+      }
+      return;
+    }
     for (int i = x0; i < x1; i++)
       for (int j = y0; j < y1; j++)
         for (int up = 0; up < 2; up++) {
