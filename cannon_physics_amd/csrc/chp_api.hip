@@ -211,7 +211,7 @@ __global__ __launch_bounds__(64) void k_big_post(const DevWorld<R>* __restrict__
 // narrowphase: per-class pair counts per world, class-major offsets, placement in the queue,
 // then one pair per thread over the thread classes and one pair per wave over the wave class,
 // each grid-wide over the whole batch.  qcls: class-major, qcls[c * n + list index]; bounds:
-// NP_NCLS + 1 ints.
+// NP_NCLS + 1 ints, then the wave classes' NP_NTICKET pair tickets.
 template <typename R>
 __global__ __launch_bounds__(64) void k_narrow_count(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
                                                      int* __restrict__ qcls, int n) {
@@ -247,11 +247,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 }
 template <typename R, int CLS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_narrow_wave(const DevWorld<R>* __restrict__ worlds, const int2* __restrict__ q,
-                                                     const int* __restrict__ bounds, double dt) {
+                                                     int* __restrict__ bounds, double dt) {
   __shared__ f3 ws[4][NPCtx<R, true>::WS_F3];            // one LDS workspace per wave
   __shared__ ContactRec<R> loc[4][WAVE_KMAX];             // and one contact buffer
   constexpr unsigned kinds = CLS == NP_WHC ? (NK_CC | NK_HC) : NK_ALL;
-  st_narrow_wave<R, kinds>(worlds, q + bounds[CLS], bounds[CLS + 1] - bounds[CLS], (R)dt, ws[threadIdx.x >> 6],
+  st_narrow_wave<R, kinds>(worlds, q + bounds[CLS], bounds[CLS + 1] - bounds[CLS], bounds + NP_NCLS + 1 + (CLS - NP_WHC),
+                           (R)dt, ws[threadIdx.x >> 6],
                            loc[threadIdx.x >> 6], WAVE_KMAX);
 }
 
@@ -1281,7 +1282,7 @@ static int ensure_queue(chp_ctx* ctx, const uint32_t* ids, uint32_t n) {
   size_t need = 1;
   for (uint32_t k = 0; k < n; k++) need += (size_t)ctx->worlds[ids[k]].pair_cap;
   if (need > ((size_t)1 << 31) - 1) return fail(ctx, CHP_E_CAPACITY, "more than 2^31 pairs in one batch");
-  size_t ncls = (size_t)n * NP_NCLS + NP_NCLS + 1;
+  size_t ncls = (size_t)n * NP_NCLS + NP_NCLS + 1 + NP_NTICKET;
   if (ctx->d_qcls_cap < ncls) {
     if (ctx->d_qcls) (void)hipFree(ctx->d_qcls);
     ctx->d_qcls = nullptr;
@@ -1769,7 +1770,7 @@ template <typename R> static int step_noalloc_t(chp_ctx* ctx, uint32_t id, doubl
   };
   const size_t nbig_ints = 2 * (size_t)nspans + 2 * (size_t)ntiles + (big ? 1 : 0);
   const size_t o_tab = take(sizeof(DevWorld<R>)), o_list = take(sizeof(int)), o_big = take(nbig_ints * sizeof(int)),
-               o_qcls = take((2 * NP_NCLS + 1) * sizeof(int)), o_q = take((size_t)S.pair_cap * sizeof(int2));
+               o_qcls = take((2 * NP_NCLS + 1 + NP_NTICKET) * sizeof(int)), o_q = take((size_t)S.pair_cap * sizeof(int2));
   if (off > NOALLOC_SCRATCH_BYTES)
     return fail(ctx, CHP_E_CAPACITY, "world " + std::to_string(id) + " needs " + std::to_string(off) +
                                          " bytes of step buffers; the no-allocation path holds " +

@@ -632,6 +632,9 @@ template <typename R> __device__ void st_narrow(const DevWorld<R>& W, R dt, f3* 
 // Wave classes: NP_WHC, single-shape convex pairs needing a wave (convex–heightfield, large SAT),
 // compiled with the convex resolvers only; NP_WAVE, the rest (compound bodies).
 enum { NP_SS = 0, NP_SP, NP_SB, NP_SC, NP_SH, NP_PC, NP_CC, NP_OTHER, NP_WHC, NP_WAVE, NP_NCLS };
+// after the class bounds: one pair ticket per wave class (NP_WHC, NP_WAVE)
+constexpr int NP_NTICKET = 2;
+static_assert(NP_WAVE == NP_WHC + 1, "the tickets follow the wave classes' order");
 template <typename R> __device__ __forceinline__ int pair_class(const DevWorld<R>& W, int bi, int bj) {
   if (W.ev_mask) {   // justTest pairs (narrow_phase.dart:665-668): their overlap probe is on the wave path
     const int ti = W.type[bi], tj = W.type[bj];
@@ -714,7 +717,10 @@ __device__ inline void st_narrow_scan(int* cnt, int n, int* bounds, int* sh) {
       if (tid == 0) bounds[c] = run;
       run += tot;
     }
-    if (tid == 0) bounds[NP_NCLS] = run;
+    if (tid == 0) {
+      bounds[NP_NCLS] = run;
+      for (int t = 0; t < NP_NTICKET; t++) bounds[NP_NCLS + 1 + t] = 0;   // the wave classes' tickets
+    }
     __syncthreads();
   }
   for (int w0 = 0; w0 < n; w0 += bd) {
@@ -771,14 +777,20 @@ __device__ void st_narrow_class(const DevWorld<R>* worlds, const int2* q, const 
 // contact records a wave buffers per pair: a heightfield pair touching several pillars has more
 // contacts than a thread's pair, and a pair over the buffer runs twice
 constexpr int WAVE_KMAX = 64;
-// the queued separating-axis pairs, one wave per pair (grid-stride over the queue)
+// the queued separating-axis pairs, one wave per pair.  A wave takes its next pair from a
+// ticket counter (zero at the launch), so that waves that drew cheap pairs go on to the rest
+// while one runs an expensive pair: the costs differ by orders of magnitude (a hull falling
+// clear of a heightfield against one resting on a dozen pillars).
 // (ws: the wave's LDS workspace; loc: its LDS contact buffer of kmax records)
 template <typename R, unsigned KINDS>
-__device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, R dt, f3* ws,
+__device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, int* ticket, R dt, f3* ws,
                                ContactRec<R>* loc, int kmax) {
   const int lane = threadIdx.x & 63;
-  const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
-  for (int i = gw; i < cnt; i += nw) {
+  for (;;) {
+    int i = 0;
+    if (lane == 0) i = atomicAdd(ticket, 1);
+    i = __shfl(i, 0, 64);
+    if (i >= cnt) break;
     const int2 q = queue[i];
     const DevWorld<R>& W = worlds[q.x];
     const int p = q.y;
