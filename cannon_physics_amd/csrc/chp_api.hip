@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                      int* __restrict__ bounds, double dt) {
   __shared__ f3 ws[4][NPCtx<R, true>::WS_F3];            // one LDS workspace per wave
   __shared__ ContactRec<R> loc[4][WAVE_KMAX];             // and one contact buffer
-  constexpr unsigned kinds = CLS == NP_WHC ? (NK_CC | NK_HC) : NK_ALL;
+  constexpr unsigned kinds = CLS == NP_WHC ? (NK_CC | NK_HC | NK_SH) : NK_ALL;
   st_narrow_wave<R, kinds>(worlds, q + bounds[CLS], bounds[CLS + 1] - bounds[CLS], bounds + NP_NCLS + 1 + (CLS - NP_WHC),
                            (R)dt, ws[threadIdx.x >> 6],
                            loc[threadIdx.x >> 6], WAVE_KMAX);
@@ -1463,7 +1463,6 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
     k_narrow_cls<R, NP_SP><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
     k_narrow_cls<R, NP_SB><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
     k_narrow_cls<R, NP_SC><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
-    k_narrow_cls<R, NP_SH><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
     k_narrow_cls<R, NP_PC><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
     k_narrow_cls<R, NP_CC><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);
     k_narrow_cls<R, NP_OTHER><<<blocks, 64, 0, st>>>(dw, V.q, bounds, dt);

@@ -727,6 +727,43 @@ template <typename R, bool WAVE = false, unsigned KINDS = NK_ALL> struct NPCtx {
     if (!hfWindow(hf, local, Rr, x0, x1, y0, y1, mn, mx)) return;
     if (ld<R>(local.z) - Rr > mx || ld<R>(local.z) + Rr < mn) return;
     SRef pref{1, -1, sj.id};   // the pillar hull (the contacts carry the override shapes)
+    if constexpr (WAVE) {
+      // The window's pillars (in the loop order i, j, lower/upper) are dealt to the lanes, 64 a
+      // round: each lane runs sphereConvex on its pillar into a record of its own, and the wave
+      // then pushes the records in loop order.  The reference's return after a cell that added
+      // more than two contacts (:1423-1427) never fires: sphereConvex adds at most one contact
+      // per pillar, two per cell.
+      const int lane = (int)(threadIdx.x & 63);
+      const int nyw = y1 - y0, ncand = (x1 - x0) * nyw * 2;
+      constexpr int NWORD = (int)(sizeof(ContactRec<R>) / sizeof(int));
+      for (int c0 = 0; c0 < ncand; c0 += 64) {
+        const int c = c0 + lane;
+        ContactRec<R> rec{};
+        int got = 0;
+        if (c < ncand) {
+          const int up = c & 1, ij = c >> 1, i = x0 + ij / nyw, j = y0 + ij % nyw;
+          const HullView<R> H = hull_view(W, pillarHull(hf, i, j, up));
+          const f3 wpo = point_to_world<R>(xj, qj, pillarOff(hf, i, j, up));
+          if (dist3<R>(xi, wpo) < H.bsr + rbsr) {
+            Sink<R, false> one{&rec, 1, 0, 0};
+            NPCtx<R, false, NK_SC> t(W, dt, &one);
+            t.cm = cm;
+            got = t.sphereConvex(Rr, H, xi, wpo, qj, bi, bj, si, pref, si, sj);
+          }
+        }
+        for (uint64_t m = __ballot(got != 0); m; m &= m - 1) {
+          const int l = __builtin_ctzll(m);
+          int w[NWORD];
+          __builtin_memcpy(w, &rec, sizeof(w));
+#pragma unroll
+          for (int k = 0; k < NWORD; k++) w[k] = __builtin_amdgcn_readlane(w[k], l);
+          ContactRec<R> o;
+          __builtin_memcpy(&o, w, sizeof(w));
+          sink->push(o);
+        }
+      }
+      return;
+    }
     for (int i = x0; i < x1; i++)
       for (int j = y0; j < y1; j++) {
         int before = sink->n;

@@ -629,8 +629,9 @@ template <typename R> __device__ void st_narrow(const DevWorld<R>& W, R dt, f3* 
 // The batched narrowphase sorts the pairs of every listed world by resolver class, so that a
 // wave of the thread kernel runs one resolver on neighbouring data.  Classes 0..NP_WAVE-1 run one
 // pair per thread, class NP_WAVE one pair per wave.
-// Wave classes: NP_WHC, single-shape convex pairs needing a wave (convex–heightfield, large SAT),
-// compiled with the convex resolvers only; NP_WAVE, the rest (compound bodies).
+// Wave classes: NP_WHC, single-shape pairs needing a wave (sphere– and convex–heightfield, large
+// SAT), compiled with those resolvers only; NP_WAVE, the rest (compound bodies).  NP_SH (sphere–
+// heightfield one pair per thread) is left empty by the batched path.
 enum { NP_SS = 0, NP_SP, NP_SB, NP_SC, NP_SH, NP_PC, NP_CC, NP_OTHER, NP_WHC, NP_WAVE, NP_NCLS };
 // after the class bounds: one pair ticket per wave class (NP_WHC, NP_WAVE)
 constexpr int NP_NTICKET = 2;
@@ -649,7 +650,7 @@ template <typename R> __device__ __forceinline__ int pair_class(const DevWorld<R
     if (hi == CHP_SHAPE_PLANE) return NP_SP;
     if (hi == CHP_SHAPE_BOX) return NP_SB;
     if (hi == CHP_SHAPE_CONVEX) return NP_SC;
-    if (hi == CHP_SHAPE_HEIGHTFIELD) return NP_SH;
+    if (hi == CHP_SHAPE_HEIGHTFIELD) return NP_WHC;   // the wave deals the pillars to its lanes
     return NP_OTHER;
   }
   if (lo == CHP_SHAPE_PLANE && (hi == CHP_SHAPE_BOX || hi == CHP_SHAPE_CONVEX)) return NP_PC;
