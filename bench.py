@@ -460,7 +460,9 @@ def main():
                      "bytes_per_launch": B_ROW_GS * E, "avg_launch_ms": gs_ms},
         "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes_per_step": b_step},
-        "dominant_kernel": dom,
+        # the stage of most time; the roofline above is k_gs's, the longest single kernel (the
+        # narrowphase stage is a dozen kernels, one per pair class)
+        "dominant_stage": dom,
         "stage_ms_per_step": {k: v / K for k, v in stages.items()},   # the profiled steps
         "plain_gpu": plain,
         "serving": serving,

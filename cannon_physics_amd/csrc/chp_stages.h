@@ -778,15 +778,18 @@ __device__ void st_narrow_class(const DevWorld<R>* worlds, const int2* q, const 
 // contact records a wave buffers per pair: a heightfield pair touching several pillars has more
 // contacts than a thread's pair, and a pair over the buffer runs twice
 constexpr int WAVE_KMAX = 64;
-// the queued separating-axis pairs, one wave per pair.  A wave takes its next pair from a
-// ticket counter (zero at the launch), so that waves that drew cheap pairs go on to the rest
-// while one runs an expensive pair: the costs differ by orders of magnitude (a hull falling
-// clear of a heightfield against one resting on a dozen pillars).
+// the queued separating-axis pairs, one wave per pair.  A wave takes each pair from a ticket
+// counter (zero at the launch) when it is free, so that waves that drew cheap pairs go on to
+// the rest while one runs an expensive pair: the costs differ by orders of magnitude (a hull
+// falling clear of a heightfield against one resting on a dozen pillars).  Waves numbered at or
+// beyond the queue's length leave at once, so an empty or short queue costs few atomics on the
+// one counter.
 // (ws: the wave's LDS workspace; loc: its LDS contact buffer of kmax records)
 template <typename R, unsigned KINDS>
 __device__ void st_narrow_wave(const DevWorld<R>* worlds, const int2* queue, int cnt, int* ticket, R dt, f3* ws,
                                ContactRec<R>* loc, int kmax) {
   const int lane = threadIdx.x & 63;
+  if ((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) >= cnt) return;
   for (;;) {
     int i = 0;
     if (lane == 0) i = atomicAdd(ticket, 1);
