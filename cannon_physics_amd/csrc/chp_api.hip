@@ -213,8 +213,8 @@ __global__ __launch_bounds__(64) void k_big_post(const DevWorld<R>* __restrict__
 // each grid-wide over the whole batch.  qcls: class-major, qcls[c * n + list index]; bounds:
 // NP_NCLS + 1 ints, then the wave classes' NP_NTICKET pair tickets.
 template <typename R>
-__global__ __launch_bounds__(64) void k_narrow_count(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                                     int* __restrict__ qcls, int n) {
+__global__ __launch_bounds__(256) void k_narrow_count(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                      int* __restrict__ qcls, int n) {
   __shared__ int sh[64];
   int* cnt = qcls + blockIdx.x;
   const DevWorld<R>& W = worlds[list[blockIdx.x]];
@@ -229,8 +229,8 @@ __global__ __launch_bounds__(1024) void k_narrow_scan(int* __restrict__ qcls, in
   st_narrow_scan(qcls, n, bounds, sh);
 }
 template <typename R>
-__global__ __launch_bounds__(64) void k_narrow(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
-                                               const int* __restrict__ qcls, int2* __restrict__ q, int n) {
+__global__ __launch_bounds__(256) void k_narrow(const DevWorld<R>* __restrict__ worlds, const int* __restrict__ list,
+                                                const int* __restrict__ qcls, int2* __restrict__ q, int n) {
   __shared__ int sh[64];
   CHP_WORLD_PROLOGUE
   if (W.nb >= BP_BIG) return;  // k_big_nplace
@@ -1452,10 +1452,12 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
   {
     const int qcap = (int)V.qcap;
     int* bounds = V.qcls + (size_t)n * NP_NCLS;
-    k_narrow_count<R><<<n, 64, 0, st>>>(dw, d_list, V.qcls, n);
+    // the per-world classification passes: 256 threads a world (a world's pairs are a few
+    // dependent loads each, so the largest worlds' loops set the time)
+    k_narrow_count<R><<<n, 256, 0, st>>>(dw, d_list, V.qcls, n);
     if (V.nw) k_big_ncount<R><<<V.nspans, 256, 0, st>>>(dw, d_list, V.spans(), V.qcls, n);
     k_narrow_scan<<<1, 1024, 0, st>>>(V.qcls, n, bounds);
-    k_narrow<R><<<n, 64, 0, st>>>(dw, d_list, V.qcls, V.q, n);
+    k_narrow<R><<<n, 256, 0, st>>>(dw, d_list, V.qcls, V.q, n);
     if (V.nw) k_big_nplace<R><<<V.nspans, 256, 0, st>>>(dw, d_list, V.spans(), V.qcls, V.q, n);
     // one kernel per thread class (each compiles only its resolver), up to 4 waves per SIMD
     const int blocks = (std::max(64, std::min(qcap, 256 * 4 * 4 * 64)) + 63) / 64;

@@ -695,12 +695,25 @@ __device__ void st_narrow_place(const DevWorld<R>& W, int wid, const int* off, i
 // by wave shuffles and one exchange of the wave totals.
 __device__ inline void st_narrow_scan(int* cnt, int n, int* bounds, int* sh) {
   const int tid = threadIdx.x, bd = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = bd >> 6;
+  // the first RC rounds' counts are loaded once, all at the start, and kept in registers for the
+  // second pass (the block is alone on its CU: each round's loads would be a round trip)
+  constexpr int RC = 4;
+  int cache[RC][NP_NCLS];
   int base[NP_NCLS];
   {
     int part[NP_NCLS];
 #pragma unroll
     for (int c = 0; c < NP_NCLS; c++) part[c] = 0;
-    for (int w = tid; w < n; w += bd)
+#pragma unroll
+    for (int r = 0; r < RC; r++) {
+      const int w = r * bd + tid;
+#pragma unroll
+      for (int c = 0; c < NP_NCLS; c++) {
+        cache[r][c] = w < n ? cnt[c * n + w] : 0;
+        part[c] += cache[r][c];
+      }
+    }
+    for (int w = RC * bd + tid; w < n; w += bd)
 #pragma unroll
       for (int c = 0; c < NP_NCLS; c++) part[c] += cnt[c * n + w];
 #pragma unroll
@@ -724,12 +737,11 @@ __device__ inline void st_narrow_scan(int* cnt, int n, int* bounds, int* sh) {
     }
     __syncthreads();
   }
-  for (int w0 = 0; w0 < n; w0 += bd) {
-    const int w = w0 + tid;
-    int v[NP_NCLS], incl[NP_NCLS];
+  // one round: the counts v of world w (one per thread) become offsets
+  auto scan_round = [&](int w, const int (&v)[NP_NCLS]) {
+    int incl[NP_NCLS];
 #pragma unroll
     for (int c = 0; c < NP_NCLS; c++) {
-      v[c] = w < n ? cnt[c * n + w] : 0;
       incl[c] = v[c];
       for (int off = 1; off < 64; off <<= 1) {
         int x = __shfl_up(incl[c], off, 64);
@@ -750,6 +762,16 @@ __device__ inline void st_narrow_scan(int* cnt, int n, int* bounds, int* sh) {
       base[c] += tot;
     }
     __syncthreads();
+  };
+#pragma unroll
+  for (int r = 0; r < RC; r++)
+    if (r * bd < n) scan_round(r * bd + tid, cache[r]);
+  for (int w0 = RC * bd; w0 < n; w0 += bd) {
+    const int w = w0 + tid;
+    int v[NP_NCLS];
+#pragma unroll
+    for (int c = 0; c < NP_NCLS; c++) v[c] = w < n ? cnt[c * n + w] : 0;
+    scan_round(w, v);
   }
 }
 
