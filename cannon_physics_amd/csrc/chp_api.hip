@@ -1480,7 +1480,10 @@ template <typename R> static int launch_step(chp_ctx* ctx, const StepView& V, in
   // records in level order (keep: also as row records, for the capture)
   const int keep = ctx->keep_rows ? 1 : 0;
   const int* qbounds = V.qcls + (size_t)n * NP_NCLS;
-  const int qblocks = (std::max(64, std::min((int)V.qcap, 256 * 4 * 4 * 64)) + 63) / 64;
+  // one queued pair per thread: a grid of up to 1M threads (a pair's rows are a chain of
+  // dependent loads, so the more pairs in flight the better; the resident waves are what the
+  // registers allow, the rest wait for a slot)
+  const int qblocks = (std::max(64, std::min((int)V.qcap, 256 * 4 * 16 * 64)) + 63) / 64;
   auto build_rows = [&]() {
     k_pair_rows<R, ROWS_GS><<<qblocks, 64, 0, st>>>(dw, V.q, qbounds, dt, keep);
     k_joint_rows<R, ROWS_GS><<<n, 64, 0, st>>>(dw, d_list, dt, keep);
