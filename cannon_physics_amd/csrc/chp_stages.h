@@ -235,6 +235,27 @@ __device__ __forceinline__ unsigned long long bp_row_chunk(const DevWorld<R>& W,
   bool hit = j < i && bp_test<R, LDS>(W, L, i, j);
   return __ballot(hit);
 }
+// body i's broadphase data from LDS, read once per row (the chunk loop stores ballot masks to
+// LDS, so the compiler could not keep these loads out of it)
+template <typename R> struct BpRowBody {
+  int group, mask, stat;
+  f3 p;
+  R r;
+  __device__ BpRowBody(const BpLds<R>& L, int i)
+      : group(L.gm[2 * i]), mask(L.gm[2 * i + 1]), stat(L.stat[i]), p(mk3(L.pos[3 * i], L.pos[3 * i + 1], L.pos[3 * i + 2])),
+        r(L.brad[i]) {}
+};
+// bp_test<R, true> (i, j) with body i's data held
+template <typename R>
+__device__ __forceinline__ unsigned long long bp_row_chunk_lds(const BpLds<R>& L, const BpRowBody<R>& A, int i, int j0) {
+  const int j = j0 + (int)(threadIdx.x & 63);
+  bool hit = false;
+  if (j < i && (A.group & L.gm[2 * j + 1]) != 0 && (L.gm[2 * j] & A.mask) != 0 && !(A.stat && L.stat[j])) {
+    const f3 pb = mk3(L.pos[3 * j], L.pos[3 * j + 1], L.pos[3 * j + 2]);
+    hit = bp_sphere_at(A.p, (double)A.r, pb, (double)L.brad[j]);
+  }
+  return __ballot(hit);
+}
 template <typename R, bool LDS> __device__ bool st_bp_naive(const DevWorld<R>& W, int* sh, const BpLds<R>& L) {
   const int n = W.nb, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   if constexpr (LDS) {
@@ -250,12 +271,15 @@ template <typename R, bool LDS> __device__ bool st_bp_naive(const DevWorld<R>& W
   }
   for (int i = wave; i < n; i += nwave) {
     int c = 0;
-    for (int j0 = 0; j0 < i; j0 += 64) {
-      const unsigned long long m = bp_row_chunk<R, LDS>(W, L, i, j0);
-      if constexpr (LDS) {
+    if constexpr (LDS) {
+      const BpRowBody<R> A(L, i);
+      for (int j0 = 0; j0 < i; j0 += 64) {
+        const unsigned long long m = bp_row_chunk_lds<R>(L, A, i, j0);
         if (lane == 0) L.masks[bp_chunks(i) + (j0 >> 6)] = m;
+        c += __popcll(m);
       }
-      c += __popcll(m);
+    } else {
+      for (int j0 = 0; j0 < i; j0 += 64) c += __popcll(bp_row_chunk<R, LDS>(W, L, i, j0));
     }
     if (lane == 0) W.cnt[i] = c;
   }
