@@ -366,7 +366,7 @@ def main():
     dev_ms = ctx.last_step_ms()
     after = stats_totals(ctx, wids)
     # the stage breakdown: the same number of steps again with per-stage events (a profiled call
-    # launches the whole batch on one stream; unprofiled batches of 64+ worlds run as two halves
+    # launches the whole batch on one stream; unprofiled batches of 16+ worlds run as two halves
     # on two streams)
     ctx.set_profiling(True)
     t1 = time.perf_counter()

@@ -1573,7 +1573,7 @@ static int step_graph(chp_ctx* ctx, const std::vector<int>& list, int nbig, int 
 // launches nsteps steps of the listed worlds (largest first); with wait, synchronises and, when
 // status is given, returns every listed world's status word in list order
 // batches of at least this many worlds run as two halves on two streams
-constexpr uint32_t SPLIT_MIN_WORLDS = 64;
+constexpr uint32_t SPLIT_MIN_WORLDS = 16;
 
 static int status_words(chp_ctx* ctx, const uint32_t* ids, uint32_t n, std::vector<int>* status) {
   std::vector<int> st;

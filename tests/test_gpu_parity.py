@@ -122,7 +122,7 @@ def test_fp32_within_tolerance_of_reference_model():
 
 
 def test_two_stream_batch_equals_one_stream():
-    """A batch of 64+ worlds runs as two halves on two streams (an unprofiled chp_step, and
+    """A batch of 16+ worlds runs as two halves on two streams (an unprofiled chp_step, and
     chp_step_async + chp_sync); a profiled call runs the same batch on one stream: every world
     ends bit-identical either way, with large (grid-wide) worlds and sleep/event worlds in it."""
     c = ctx("f32")
