@@ -272,7 +272,9 @@ int chp_step_plain(chp_ctx* ctx, uint32_t world_id, double dt, int32_t nsteps);
  * buffers (chiefly 8 bytes per pair of its pair capacity) exceed it is refused with
  * CHP_E_CAPACITY.  Calls on one device share the array; calls that overlap on one device race. */
 int chp_step_world_noalloc(chp_ctx* ctx, uint32_t world_id, double dt, int32_t nsteps);
-/* Asynchronous batched step on the context's stream; chp_sync waits for it. */
+/* Asynchronous batched step on the context's streams (a batch of 16 or more worlds steps as two
+ * halves, one on each of the context's two streams, joined at the end of the call); chp_sync
+ * waits for it. */
 int chp_step_async(chp_ctx* ctx, const uint32_t* worlds, uint32_t nworlds, double dt,
                    int32_t nsteps);
 int chp_sync(chp_ctx* ctx);
