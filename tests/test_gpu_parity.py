@@ -149,6 +149,27 @@ def test_two_stream_batch_equals_one_stream():
             c.destroy_world(w)
 
 
+def test_two_stream_heightfield_batch_equals_one_stream():
+    """A batch of 18 heightfield worlds (spheres, boxes and hulls over a terrain: the wave
+    narrowphase classes and their pair tickets, one set per half) ends bit-identical on two
+    streams and on one."""
+    c = ctx("f32")
+    recs = [scenes.c5_terrain_world(900 + i, n=24 + 3 * i).recipe() for i in range(18)]
+    two = [c.create_world(r) for r in recs]
+    one = [c.create_world(r) for r in recs]
+    try:
+        c.step(two, DT, 40)
+        c.set_profiling(True)
+        c.step(one, DT, 40)
+        c.set_profiling(False)
+        for a, b in zip(two, one):
+            assert all(same_state(state_to_numpy(*c.download(a)), state_to_numpy(*c.download(b))).values())
+            assert c.stats(a)["sum_contacts"] == c.stats(b)["sum_contacts"] > 0
+    finally:
+        for w in two + one:
+            c.destroy_world(w)
+
+
 def test_batch_invariance_and_plain_equals_batched():
     """A world gives bit-identical results alone, in a batch of mixed worlds, and on the plain
     one-world-per-launch path."""
